@@ -1,0 +1,143 @@
+/*
+ * rca_das.h — C ABI of the MI355X (gfx950) RCA delay-and-sum I/Q beamformer.
+ *
+ * Plain C: device pointers, sizes and scalars; no torch types.  Every call is
+ * asynchronous on the given HIP stream and allocates no device memory.
+ * Calls on distinct streams are re-entrant.  Errors return a non-zero
+ * rca_status and leave a message in rca_last_error() (thread-local).
+ *
+ * Layouts (all C-order, complex = interleaved float re/im):
+ *   iq       complex64 [nFrame][nTx][nChannel][nSample]   (RCA_IQ_C64)
+ *            or half2  [nFrame][nTx][nChannel][nSample] with a per-trace
+ *            float scale [nFrame][nTx][nChannel]           (RCA_IQ_F16X2)
+ *            == the reference's data [2*nSample, nChannel, nTx, nFrame]
+ *            column-major buffer (bfVirtualSourceGeneric_RCA.m:576-578).
+ *   out      complex64 [nFrame][1 | nTx][nPix]  (1 when RCA_FLAG_COMPOUND)
+ *            == the reference's dataBf [2*nPix, nTx|1, nFrame]
+ *            (bfVirtualSourceGeneric_RCA.cu:230, .m:654-658).
+ *   mask     float [1 | nTx][nPix] (sum of Hann weights, .cu:301-304).
+ */
+#ifndef RCA_DAS_H
+#define RCA_DAS_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  RCA_OK = 0,
+  RCA_ERR_INVALID = 1,     /* bad argument (null pointer, size, flag) */
+  RCA_ERR_UNSUPPORTED = 2, /* shape outside what the kernels handle */
+  RCA_ERR_HIP = 3,         /* a HIP runtime call failed */
+  RCA_ERR_BUDGET = 4       /* a request exceeds the serving memory budget */
+} rca_status;
+
+typedef enum {
+  RCA_INTERP_QUADRATIC = 0,   /* reference default (.m:159-161; .cu:355-365) */
+  RCA_INTERP_LANCZOS_REF = 1, /* reference "Lanczos 5-lobes" (.cu:331-350) */
+  RCA_INTERP_LANCZOS3 = 2,    /* textbook Lanczos a=3, 6 taps */
+  RCA_INTERP_LANCZOS4 = 3     /* textbook Lanczos a=4, 8 taps */
+} rca_interp;
+
+typedef enum { RCA_IQ_C64 = 0, RCA_IQ_F16X2 = 1 } rca_iq_dtype;
+
+enum {
+  RCA_FLAG_COMPOUND = 1,   /* sum over tilts in the kernel (.m:690-697) */
+  RCA_FLAG_ACCUMULATE = 2  /* out += result (adds an RCA half, script_All.m:382) */
+};
+
+typedef enum { RCA_ABLAW_NONE = 0, RCA_ABLAW_CHANNEL = 1, RCA_ABLAW_TX_CHANNEL = 2 } rca_ablaw;
+
+/*
+ * Beamforming geometry and parameters of one RCA half: the argument list of
+ * the reference kernel (bfVirtualSourceGeneric_RCA.cu:103-133) with device
+ * pointers, grouped so it can stay resident between calls.
+ */
+typedef struct {
+  const float* grid_x;       /* [nPix] gridBfX (m) */
+  const float* grid_y;       /* [nPix] gridBfY */
+  const float* grid_z;       /* [nPix] gridBfZ */
+  const float* ele_x;        /* [nChannel] centred posEleX */
+  const float* ele_y;        /* [nChannel] centred posEleY */
+  const float* ele_z;        /* [nChannel] centred posEleZ */
+  const float* vs_x;         /* [nTx] virtualSourceX */
+  const float* vs_y;         /* [nTx] virtualSourceY */
+  const float* vs_z;         /* [nTx] virtualSourceZ */
+  const float* min_dist;     /* [nTx] minDistVirtualSource */
+  const float* ab_law_rx;    /* receive aberration delays (s) or NULL */
+  int ab_law_mode;           /* rca_ablaw: [nChannel] or [nTx][nChannel] */
+  int n_pix;
+  int n_channel;
+  int n_tx;
+  int n_sample;              /* I/Q samples per trace */
+  int receive_x;             /* flagReceiveX */
+  int mask_transmit;         /* flagMaskTransmit */
+  int interp;                /* rca_interp */
+  float aperture_x, aperture_y, aperture_z;
+  float sampling_frequency;
+  float demodulation_frequency;
+  float speed_of_sound;
+  float start_time;
+  float fnumber;
+} rca_das_desc;
+
+/*
+ * Reference kernel ABI.  Same arguments, order and meaning as
+ * bfVirtualSourceGeneric_RCA (MATLAB/lib/beamformer/RCA/bfVirtualSourceGeneric_RCA.cu:103-133,
+ * launched by feval at bfVirtualSourceGeneric_RCA.m:670-686), all pointers on
+ * the device, plus the stream.  Writes the per-tilt dataBf
+ * [2*nPix*nTx*nFrame] and mask [nPix*nTx]; the caller sums over tilts when
+ * compounding, as the reference wrapper does.  Pixels outside the transmit
+ * cone are written as zero (the reference leaves them untouched).
+ */
+int bfVirtualSourceGeneric_RCA(float* dataBf, float* mask, const float* data,
+                               const float* gridBfX, const float* gridBfY, const float* gridBfZ,
+                               const float* posEleX, const float* posEleY, const float* posEleZ,
+                               const float* virtualSourceX, const float* virtualSourceY,
+                               const float* virtualSourceZ, const float* minDistVirtualSource,
+                               int flagReceiveX, float apertureX, float apertureY, float apertureZ,
+                               float samplingFrequency, float demodulationFrequency,
+                               float speedOfSound, float startTime, float fnumber, int nPix,
+                               int nChannel, int nTx, int nFrame, int nSample,
+                               int flagInterpolationLanczos, int flagMaskTransmit,
+                               hipStream_t stream);
+
+/*
+ * Forward beamforming of nFrame frames of one RCA half.
+ * Replaces the feval + tilt-sum + gather sequence of
+ * bfVirtualSourceGeneric_RCA.m:664-711.  trace_scale is required for
+ * RCA_IQ_F16X2 and ignored for RCA_IQ_C64.  mask may be NULL.
+ */
+int rca_forward(const rca_das_desc* desc, const void* iq, int iq_dtype, const float* trace_scale,
+                int n_frame, void* out, float* mask, int flags, hipStream_t stream);
+
+/*
+ * Adjoint (no reference counterpart; serves autograd through the forward):
+ * grad_iq = A^H grad_out, complex64 [nFrame][nTx][nChannel][nSample].
+ * grad_out is complex64 [nFrame][grad_tx][nPix] with grad_tx in {1, nTx}
+ * (1 = gradient of the compounded output).  grad_iq is overwritten unless
+ * RCA_FLAG_ACCUMULATE is set.
+ */
+int rca_adjoint(const rca_das_desc* desc, const void* grad_out, int grad_tx, int n_frame,
+                void* grad_iq, int flags, hipStream_t stream);
+
+/*
+ * Launch-shape override for tuning sweeps (no reference counterpart):
+ * voxels per lane v in {1,2,4}, frames per workgroup fb in {1,2,4,8}, adjoint
+ * voxel splits; 0 restores the automatic choice.  Process-wide.
+ */
+int rca_set_tuning(int v, int fb, int split);
+
+/* Last error message of the calling thread ("" when none). */
+const char* rca_last_error(void);
+
+/* Library version string. */
+const char* rca_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RCA_DAS_H */

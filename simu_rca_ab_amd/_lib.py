@@ -1,0 +1,79 @@
+"""ctypes binding of the C ABI declared in include/rca_das.h.
+
+The shared library is built in-tree (``simu_rca_ab_amd/lib/librca_das.so``) by
+``simu_rca_ab_amd.build.build_library``.  There is no fallback: if the library
+is missing, importing the binding raises, so nothing can silently run on a CPU
+path.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "librca_das.so"
+
+RCA_OK, RCA_ERR_INVALID, RCA_ERR_UNSUPPORTED, RCA_ERR_HIP, RCA_ERR_BUDGET = range(5)
+INTERP = {"quadratic": 0, "lanczos": 1, "lanczos3": 2, "lanczos4": 3}
+IQ_C64, IQ_F16X2 = 0, 1
+FLAG_COMPOUND, FLAG_ACCUMULATE = 1, 2
+ABLAW = {None: 0, "channel": 1, "tx_channel": 2}
+
+# Every symbol include/rca_das.h declares.
+EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last_error",
+           "rca_version", "rca_set_tuning")
+
+
+class RcaDasDesc(ctypes.Structure):
+    """Mirror of ``rca_das_desc``."""
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "grid_x", "grid_y", "grid_z", "ele_x", "ele_y", "ele_z", "vs_x", "vs_y", "vs_z",
+        "min_dist", "ab_law_rx")] + [(n, ctypes.c_int) for n in (
+        "ab_law_mode", "n_pix", "n_channel", "n_tx", "n_sample", "receive_x", "mask_transmit",
+        "interp")] + [(n, ctypes.c_float) for n in (
+        "aperture_x", "aperture_y", "aperture_z", "sampling_frequency",
+        "demodulation_frequency", "speed_of_sound", "start_time", "fnumber")]
+
+
+class RcaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rca_das error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"HIP library not built: {LIB_PATH}. Run "
+                          "`python -c 'import __graft_entry__ as g; g.build()'` first.")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    dp = ctypes.POINTER(RcaDasDesc)
+    lib.rca_forward.argtypes = [dp, vp, i, vp, i, vp, vp, i, vp]
+    lib.rca_adjoint.argtypes = [dp, vp, i, i, vp, i, vp]
+    lib.rca_set_tuning.argtypes = [i, i, i]
+    lib.bfVirtualSourceGeneric_RCA.argtypes = [vp] * 13 + [i, f, f, f, f, f, f, f, f] + [i] * 7 + [vp]
+    for name in ("rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA"):
+        getattr(lib, name).restype = ctypes.c_int
+    lib.rca_last_error.restype = ctypes.c_char_p
+    lib.rca_version.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(rc: int):
+    if rc != RCA_OK:
+        raise RcaError(rc, load().rca_last_error().decode())
+
+
+def version() -> str:
+    return load().rca_version().decode()
+
+
+def set_tuning(v: int = 0, fb: int = 0, split: int = 0):
+    """Override the automatic launch shape (0 = automatic)."""
+    check(load().rca_set_tuning(int(v), int(fb), int(split)))

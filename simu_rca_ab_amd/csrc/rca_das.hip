@@ -1,0 +1,807 @@
+// RCA delay-and-sum I/Q beamformer for MI355X (gfx950).
+//
+// What is computed is the reference kernel
+// MATLAB/lib/beamformer/RCA/bfVirtualSourceGeneric_RCA.cu:103-391: for every
+// voxel p, tilt t and receive line c,
+//   tau  = (|P - VS_t|_(receive-axis zeroed) - d0_t)/c + hypot(A, z - ez_c)/c + t0
+//   u    = tau * fs,  m = floor(u),  delta = u - m
+//   out += w(A/z*F) * sum_k q_k(delta) * IQ[t][c][m+k] * exp(+j 2 pi fd (tau - 2 z / c))
+// with A = |x - ex_c| (receive on X) or |y - ey_c|, a Hann weight inside the
+// f-number aperture A < z/F, and quadratic or Lanczos weights q_k.
+//
+// How it is computed differs from the reference (one thread per
+// voxel/tilt/frame doing global gathers, fast-math FP32 geometry):
+//   * A workgroup of 256 lanes owns 256*V consecutive voxels (a depth run:
+//     the grid is depth-fastest) and FB frames.  It walks tilt -> line; for
+//     each (tilt, line) the FB I/Q traces are staged once in LDS and every
+//     lane gathers its 2a interpolation taps from LDS.
+//   * Geometry is FP64 (full-rate on CDNA4 vector ALUs): transmit delay per
+//     (voxel, tilt), receive distance per update.  The sample position u and
+//     the demodulation phase are range-reduced in FP64 (u - floor(u),
+//     p - rint(p) in cycles, with the 4 pi fd z / c term cancelled
+//     analytically) before anything is rounded to FP32, so FP32 carries only
+//     quantities in [0,1).  Interpolation weights, the rotation and the
+//     accumulation are FP32.
+//   * Lanczos weights use one sincos per update (trig identities for the
+//     per-tap sin terms) instead of 2a sinpi calls.
+//   * The tilt sum (compounding) happens in registers; one output write per
+//     voxel and frame.
+//   * FP16-I/Q traces (half2 + FP32 per-trace scale) halve LDS and HBM bytes;
+//     the math stays FP32 (mixed FMA).
+//
+// The adjoint (transposed interpolation) scatters into an LDS copy of each
+// trace with LDS float atomics and flushes once per block.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/rca_das.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(RCA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+constexpr int kThreads = 256;
+constexpr int kMaxLds = 64 * 1024;
+
+// Per-launch arguments (kernarg).  Geometry pointers come from rca_das_desc;
+// the FP64 scalars are derived on the host from the FP32 parameters.
+struct KArgs {
+  const float *gx, *gy, *gz, *ex, *ey, *ez, *vx, *vy, *vz, *d0, *ab;
+  const void* iq;
+  const float* scale;
+  float2* out;
+  float* mask;
+  const float2* gout;  // adjoint input
+  float2* gin;         // adjoint output
+  double kfs, kfd;     // fs/c, fd/c
+  double ut0, pt0;     // fs*t0, fd*t0
+  double fs, fd;
+  float apx, apy, apz, fnum;
+  int n_pix, n_ch, n_tx, n_sample, n_frame;
+  int receive_x, mask_tx, ab_mode, compound, accumulate, vec_ok;
+  int gout_tx, n_split, atomic_flush;
+};
+
+// ----------------------------------------------------------------------------
+// Device math.
+// ----------------------------------------------------------------------------
+
+// sqrt in FP64 from the hardware reciprocal square root and a Newton pair
+// (Goldschmidt form); x > 0.
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
+// sin(pi x), cos(pi x) for |x| <= 1/4 (Taylor to x^9 / x^10; |error| < 3e-8).
+__device__ __forceinline__ void sincospi_small(float x, float& s, float& c) {
+  const float t = x * 3.14159265358979f;
+  const float t2 = t * t;
+  s = t * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.75573192e-6f, -1.98412698e-4f), 8.33333333e-3f),
+                        -1.66666667e-1f), 1.0f);
+  c = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.75573192e-7f, 2.48015873e-5f),
+                                      -1.38888889e-3f), 4.16666667e-2f), -0.5f), 1.0f);
+}
+
+// sin/cos of 2*pi*p for p in [-1/2, 1/2]: quarter-turn reduction, then the
+// small-angle polynomials.
+__device__ __forceinline__ void sincos2pi(float p, float& s, float& c) {
+  const float y = 2.0f * p;
+  const float q = rintf(2.0f * y);
+  const float r = fmaf(-0.5f, q, y);
+  float sr, cr;
+  sincospi_small(r, sr, cr);
+  const int qi = ((int)q) & 3;
+  s = qi == 0 ? sr : qi == 1 ? cr : qi == 2 ? -sr : -cr;
+  c = qi == 0 ? cr : qi == 1 ? -sr : qi == 2 ? -cr : sr;
+}
+
+// Hann weight 0.5*(1 + cos(pi*rho)) = cos^2(pi*rho/2) for rho in [0, 1).
+__device__ __forceinline__ float hann(float rho) {
+  const float t = rho * 1.57079632679490f;
+  const float t2 = t * t;
+  float c = fmaf(t2, -1.14707456e-11f, 2.08767570e-9f);
+  c = fmaf(t2, c, -2.75573192e-7f);
+  c = fmaf(t2, c, 2.48015873e-5f);
+  c = fmaf(t2, c, -1.38888889e-3f);
+  c = fmaf(t2, c, 4.16666667e-2f);
+  c = fmaf(t2, c, -0.5f);
+  c = fmaf(t2, c, 1.0f);
+  return c * c;
+}
+
+// cos(pi j / W), sin(pi j / W) for the Lanczos tap identities.
+__host__ __device__ constexpr float cs_cos(int W, int j) {
+  const int m = ((j % (2 * W)) + 2 * W) % (2 * W);
+  if (W == 2) return m == 0 ? 1.f : m == 1 ? 0.f : m == 2 ? -1.f : 0.f;
+  if (W == 3) return m == 0 ? 1.f : m == 1 ? 0.5f : m == 2 ? -0.5f : m == 3 ? -1.f : m == 4 ? -0.5f : 0.5f;
+  // W == 4
+  return m == 0 ? 1.f : m == 1 ? 0.70710678118655f : m == 2 ? 0.f : m == 3 ? -0.70710678118655f
+       : m == 4 ? -1.f : m == 5 ? -0.70710678118655f : m == 6 ? 0.f : 0.70710678118655f;
+}
+__host__ __device__ constexpr float cs_sin(int W, int j) {
+  const int m = ((j % (2 * W)) + 2 * W) % (2 * W);
+  if (W == 2) return m == 0 ? 0.f : m == 1 ? 1.f : m == 2 ? 0.f : -1.f;
+  if (W == 3) return m == 0 ? 0.f : m <= 2 ? 0.86602540378444f : m == 3 ? 0.f : -0.86602540378444f;
+  return m == 0 ? 0.f : m == 1 ? 0.70710678118655f : m == 2 ? 1.f : m == 3 ? 0.70710678118655f
+       : m == 4 ? 0.f : m == 5 ? -0.70710678118655f : m == 6 ? -1.f : -0.70710678118655f;
+}
+
+// Interpolation kinds: taps k = -LO..HI around m = floor(u); W = window divisor.
+template <int KIND> struct Interp;
+template <> struct Interp<RCA_INTERP_QUADRATIC> { static constexpr int LO = 1, HI = 1, W = 0; };
+template <> struct Interp<RCA_INTERP_LANCZOS_REF> { static constexpr int LO = 2, HI = 3, W = 2; };
+template <> struct Interp<RCA_INTERP_LANCZOS3> { static constexpr int LO = 2, HI = 3, W = 3; };
+template <> struct Interp<RCA_INTERP_LANCZOS4> { static constexpr int LO = 3, HI = 4, W = 4; };
+
+// Weights q[k+LO] = sinc(d-k) * sinc((d-k)/W), d in [0, 1].
+//
+// With e = d - r (r = nearest of 0/1, |e| <= 1/2) every tap abscissa is
+// x = e - j for an integer j, so sin(pi x) = (-1)^j sin(pi e) and
+// sin(pi x / W) = sin(pi e/W) cos(pi j/W) - cos(pi e/W) sin(pi j/W):
+// one small-angle sincos serves all taps, and the tap nearest to u (j = 0)
+// is evaluated from e directly (series near 0), which keeps every weight
+// accurate to a few ulp even when d rounds to 0 or 1.
+template <int KIND>
+__device__ __forceinline__ void interp_weights(float d, float* q) {
+  constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI;
+  if constexpr (KIND == RCA_INTERP_QUADRATIC) {
+    // bfVirtualSourceGeneric_RCA.cu:355-357
+    q[0] = d * (d - 1.0f) * 0.5f;
+    q[1] = -(d - 1.0f) * (d + 1.0f);
+    q[2] = d * (d + 1.0f) * 0.5f;
+  } else {
+    constexpr int W = Interp<KIND>::W;
+    constexpr float kPi2 = 9.86960440108936f;
+    const bool up = d >= 0.5f;
+    const float e = up ? d - 1.0f : d;
+    float se, ce;
+    sincospi_small(e * (1.0f / (float)W), se, ce);
+    float S;  // sin(pi e) from the multiple-angle identity
+    if constexpr (W == 2) S = 2.0f * se * ce;
+    else if constexpr (W == 3) S = se * fmaf(-4.0f * se, se, 3.0f);
+    else S = 4.0f * se * ce * fmaf(-2.0f * se, se, 1.0f);
+    const float K = S * ((float)W / kPi2);
+    constexpr int J0 = -LO - 1;
+    constexpr int NJ = LO + HI + 2;
+    float wj[NJ];
+#pragma unroll
+    for (int i = 0; i < NJ; ++i) {
+      const int j = J0 + i;
+      if (j == 0) {
+        const float e2 = e * e;
+        constexpr float a2 = kPi2 / 6.0f * (1.0f + 1.0f / (W * W));
+        constexpr float a4 = kPi2 * kPi2 * (1.0f / 120.0f + 1.0f / (120.0f * W * W * W * W) +
+                                            1.0f / (36.0f * W * W));
+        const float ser = fmaf(e2, fmaf(e2, a4, -a2), 1.0f);
+        const float rat = K * se * __builtin_amdgcn_rcpf(e2);
+        wj[i] = fabsf(e) < 1e-3f ? ser : rat;
+      } else {
+        const float x = e - (float)j;
+        const float cj = cs_cos(W, j), sj = cs_sin(W, j);
+        const float num = se * cj - ce * sj;
+        const float sg = (j & 1) ? -K : K;
+        wj[i] = sg * num * __builtin_amdgcn_rcpf(x * x);
+      }
+    }
+#pragma unroll
+    for (int k = -LO; k <= HI; ++k) q[k + LO] = up ? wj[k - 1 - J0] : wj[k - J0];
+  }
+}
+
+// Transmit insonification cone (bfVirtualSourceGeneric_RCA.cu:85-98, 191-214),
+// in FP64.
+__device__ __forceinline__ double angle_ap(float ax, float ay, float bx, float by, float cx,
+                                           float cy) {
+  const double abx = (double)bx - ax, aby = (double)by - ay;
+  const double acx = (double)cx - ax, acy = (double)cy - ay;
+  return acos((abx * acx + aby * acy) / sqrt((abx * abx + aby * aby) * (acx * acx + acy * acy)));
+}
+
+__device__ __forceinline__ bool tx_mask(const KArgs& a, float vx, float vy, float vz, float px,
+                                        float py, float pz) {
+  const float hx = 0.5f * a.apx, hy = 0.5f * a.apy, az = a.apz;
+  const double coneX = angle_ap(vx, vz, -hx, -az, hx, -az);
+  const double coneY = angle_ap(vy, vz, -hy, -az, hy, -az);
+  return angle_ap(vx, vz, -hx, -az, px, pz) <= coneX && angle_ap(vx, vz, hx, -az, px, pz) <= coneX &&
+         angle_ap(vy, vz, -hy, -az, py, pz) <= coneY && angle_ap(vy, vz, hy, -az, py, pz) <= coneY;
+}
+
+// Sample access: complex64 or half2 (the per-trace scale is applied to the
+// rotation weights by the caller).
+__device__ __forceinline__ float2 ld_sample(const float2* s, int i) { return s[i]; }
+__device__ __forceinline__ float2 ld_sample(const __half2* s, int i) { return __half22float2(s[i]); }
+
+// Cooperative copy of nf traces (frame stride fstride samples) into LDS.
+template <typename S>
+__device__ __forceinline__ void stage(S* __restrict__ dst, const S* __restrict__ src,
+                                      long long fstride, int nf, int ns, bool vec) {
+  if (vec) {
+    const int n16 = (int)((long long)ns * sizeof(S) / 16);
+    for (int fi = 0; fi < nf; ++fi) {
+      const uint4* s = reinterpret_cast<const uint4*>(src + fi * fstride);
+      uint4* d = reinterpret_cast<uint4*>(dst + (long long)fi * ns);
+      for (int i = threadIdx.x; i < n16; i += kThreads) d[i] = s[i];
+    }
+  } else {
+    for (int fi = 0; fi < nf; ++fi)
+      for (int i = threadIdx.x; i < ns; i += kThreads) dst[fi * ns + i] = src[fi * fstride + i];
+  }
+}
+
+// Geometry of one (voxel, tilt, line) update shared by forward and adjoint.
+struct Upd {
+  float w;   // Hann weight (0 outside the aperture)
+  int m;     // floor(u)
+  float d;   // u - m
+  float p;   // demodulation phase in cycles, reduced to [-1/2, 1/2]
+  bool in_ap, in_rng;
+};
+
+template <int LO, int HI>
+__device__ __forceinline__ Upd update_geom(const KArgs& a, double px, double py, double pz,
+                                           double apF, double iapF, double Bu, double Bp,
+                                           double exd, double eyd, double ezd, double uab,
+                                           double pab) {
+  Upd r;
+  const double A = a.receive_x ? fabs(px - exd) : fabs(py - eyd);   // .cu:284-291
+  r.in_ap = A < apF;                                                // .cu:294
+  r.in_rng = false;
+  r.w = 0.0f;
+  r.m = 0;
+  r.d = 0.0f;
+  r.p = 0.0f;
+  if (!r.in_ap) return r;
+  r.w = hann((float)(A * iapF));                                     // .cu:299
+  const double dz = pz - ezd;
+  const double R = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));       // .cu:308-309
+  const double u = fma(a.kfs, R, Bu + uab);                          // .cu:310-311
+  const double mf = floor(u);                                        // .cu:319
+  r.in_rng = mf >= (double)LO && mf < (double)(a.n_sample - 1 - HI); // .cu:323
+  if (!r.in_rng) return r;
+  const double ph = fma(a.kfd, R, Bp + pab);                         // .cu:371-373
+  r.m = (int)mf;
+  r.d = (float)(u - mf);
+  r.p = (float)(ph - rint(ph));
+  return r;
+}
+
+// Per-(voxel, tilt) transmit terms: Bu = fs*((|P-VS| - d0)/c + t0),
+// Bp = fd*((|P-VS| - d0 - 2 z)/c + t0).
+__device__ __forceinline__ void tx_terms(const KArgs& a, double px, double py, double pz,
+                                         double vx, double vy, double vz, double d0, double& Bu,
+                                         double& Bp) {
+  const double dl = a.receive_x ? (py - vy) : (px - vx);             // .cu:256-269
+  const double dz = pz - vz;
+  const double tt = sqrt_nr(fma(dl, dl, dz * dz)) - d0;
+  Bu = fma(a.kfs, tt, a.ut0);
+  Bp = fma(a.kfd, fma(-2.0, pz, tt), a.pt0);
+}
+
+__device__ __forceinline__ void ab_terms(const KArgs& a, int t, int c, double& uab, double& pab) {
+  uab = 0.0;
+  pab = 0.0;
+  if (a.ab_mode) {
+    const double ab = a.ab[a.ab_mode == RCA_ABLAW_CHANNEL ? c : t * a.n_ch + c];
+    uab = a.fs * ab;
+    pab = a.fd * ab;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Forward kernel.
+// ----------------------------------------------------------------------------
+template <int KIND, int V, int FB, typename S>
+__global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
+  constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  S* tr = reinterpret_cast<S*>(smem_raw);
+  const S* iq = reinterpret_cast<const S*>(a.iq);
+
+  const int tid = threadIdx.x;
+  const long long pblk = (long long)blockIdx.x * (kThreads * V);
+  const int f0 = blockIdx.y * FB;
+  const int nf = min(FB, a.n_frame - f0);
+  const int ns = a.n_sample;
+  const long long fstride = (long long)a.n_tx * a.n_ch * ns;
+
+  double px[V], py[V], pz[V], apF[V], iapF[V];
+  bool valid[V];
+  float2 acc[V][FB];
+  float macc[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const long long p = pblk + v * kThreads + tid;
+    valid[v] = p < a.n_pix;
+    const long long pc = valid[v] ? p : 0;
+    px[v] = a.gx[pc];
+    py[v] = a.gy[pc];
+    pz[v] = a.gz[pc];
+    apF[v] = pz[v] / (double)a.fnum;   // .cu:174
+    iapF[v] = (double)a.fnum / pz[v];
+    macc[v] = 0.0f;
+#pragma unroll
+    for (int fi = 0; fi < FB; ++fi) acc[v][fi] = make_float2(0.0f, 0.0f);
+  }
+
+  for (int t = 0; t < a.n_tx; ++t) {
+    const float vxf = a.vx[t], vyf = a.vy[t], vzf = a.vz[t];
+    const double d0 = a.d0[t];
+    double Bu[V], Bp[V];
+    bool act[V];
+    float2 accT[V][FB];
+    float mT[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      tx_terms(a, px[v], py[v], pz[v], vxf, vyf, vzf, d0, Bu[v], Bp[v]);
+      act[v] = valid[v] && (!a.mask_tx || tx_mask(a, vxf, vyf, vzf, (float)px[v], (float)py[v],
+                                                   (float)pz[v]));
+      mT[v] = 0.0f;
+#pragma unroll
+      for (int fi = 0; fi < FB; ++fi) accT[v][fi] = make_float2(0.0f, 0.0f);
+    }
+
+    for (int c = 0; c < a.n_ch; ++c) {
+      __syncthreads();
+      stage<S>(tr, iq + ((long long)f0 * a.n_tx * a.n_ch + (long long)t * a.n_ch + c) * ns,
+               fstride, nf, ns, a.vec_ok);
+      __syncthreads();
+      const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
+      double uab, pab;
+      ab_terms(a, t, c, uab, pab);
+      float sc[FB];
+#pragma unroll
+      for (int fi = 0; fi < FB; ++fi) {
+        sc[fi] = 1.0f;
+        if constexpr (sizeof(S) == 4) {
+          if (fi < nf) sc[fi] = a.scale[((long long)(f0 + fi) * a.n_tx + t) * a.n_ch + c];
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        if (!act[v]) continue;
+        const Upd g = update_geom<LO, HI>(a, px[v], py[v], pz[v], apF[v], iapF[v], Bu[v], Bp[v],
+                                          exd, eyd, ezd, uab, pab);
+        mT[v] += g.w;   // .cu:301-304 (inside the aperture, before the range test)
+        if (!g.in_rng) continue;
+        float q[NT];
+        interp_weights<KIND>(g.d, q);
+        float sn, cs;
+        sincos2pi(g.p, sn, cs);
+        const float wr = g.w * cs, wi = g.w * sn;
+        const int base = g.m - LO;
+#pragma unroll
+        for (int fi = 0; fi < FB; ++fi) {
+          if (fi < nf) {
+            const S* trf = tr + fi * ns + base;
+            float re = 0.0f, im = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+              const float2 s = ld_sample(trf, k);
+              re = fmaf(q[k], s.x, re);
+              im = fmaf(q[k], s.y, im);
+            }
+            const float r2 = wr * sc[fi], i2 = wi * sc[fi];
+            accT[v][fi].x = fmaf(re, r2, fmaf(-im, i2, accT[v][fi].x));
+            accT[v][fi].y = fmaf(re, i2, fmaf(im, r2, accT[v][fi].y));
+          }
+        }
+      }
+    }
+
+    if (a.compound) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        macc[v] += mT[v];
+#pragma unroll
+        for (int fi = 0; fi < FB; ++fi) {
+          acc[v][fi].x += accT[v][fi].x;
+          acc[v][fi].y += accT[v][fi].y;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        if (!valid[v]) continue;
+        const long long p = pblk + v * kThreads + tid;
+#pragma unroll
+        for (int fi = 0; fi < FB; ++fi) {
+          if (fi < nf) {
+            float2* o = a.out + ((long long)(f0 + fi) * a.n_tx + t) * a.n_pix + p;
+            float2 r = accT[v][fi];
+            if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
+            *o = r;
+          }
+        }
+        if (a.mask && blockIdx.y == 0) {
+          float* mk = a.mask + (long long)t * a.n_pix + p;
+          *mk = a.accumulate ? *mk + mT[v] : mT[v];
+        }
+      }
+    }
+  }
+
+  if (a.compound) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      if (!valid[v]) continue;
+      const long long p = pblk + v * kThreads + tid;
+#pragma unroll
+      for (int fi = 0; fi < FB; ++fi) {
+        if (fi < nf) {
+          float2* o = a.out + (long long)(f0 + fi) * a.n_pix + p;
+          float2 r = acc[v][fi];
+          if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
+          *o = r;
+        }
+      }
+      if (a.mask && blockIdx.y == 0) {
+        float* mk = a.mask + p;
+        *mk = a.accumulate ? *mk + macc[v] : macc[v];
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Adjoint kernel: one block per (line, tilt, voxel split, frame block).
+// ----------------------------------------------------------------------------
+template <int KIND, int V, int FB>
+__global__ void __launch_bounds__(kThreads) das_adj(const KArgs a) {
+  constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* acc = reinterpret_cast<float*>(smem_raw);   // [FB][ns][2]
+  const int c = blockIdx.x, t = blockIdx.y;
+  const int split = blockIdx.z % a.n_split;
+  const int f0 = (blockIdx.z / a.n_split) * FB;
+  const int nf = min(FB, a.n_frame - f0);
+  const int ns = a.n_sample;
+  const int tid = threadIdx.x;
+
+  for (int i = tid; i < FB * ns * 2; i += kThreads) acc[i] = 0.0f;
+  __syncthreads();
+
+  const long long per = ((long long)a.n_pix + a.n_split - 1) / a.n_split;
+  const long long p0 = per * split;
+  const long long p1 = min((long long)a.n_pix, p0 + per);
+  const float vxf = a.vx[t], vyf = a.vy[t], vzf = a.vz[t];
+  const double d0 = a.d0[t];
+  const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
+  double uab, pab;
+  ab_terms(a, t, c, uab, pab);
+  const int gt = a.gout_tx == 1 ? 0 : t;
+
+  for (long long pb = p0; pb < p1; pb += (long long)kThreads * V) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const long long p = pb + (long long)v * kThreads + tid;
+      if (p >= p1) continue;
+      const double px = a.gx[p], py = a.gy[p], pz = a.gz[p];
+      if (a.mask_tx && !tx_mask(a, vxf, vyf, vzf, (float)px, (float)py, (float)pz)) continue;
+      double Bu, Bp;
+      tx_terms(a, px, py, pz, vxf, vyf, vzf, d0, Bu, Bp);
+      const Upd g = update_geom<LO, HI>(a, px, py, pz, pz / (double)a.fnum,
+                                        (double)a.fnum / pz, Bu, Bp, exd, eyd, ezd, uab, pab);
+      if (!g.in_ap || !g.in_rng) continue;
+      float q[NT];
+      interp_weights<KIND>(g.d, q);
+      float sn, cs;
+      sincos2pi(g.p, sn, cs);
+      const float wr = g.w * cs, wi = g.w * sn;
+#pragma unroll
+      for (int fi = 0; fi < FB; ++fi) {
+        if (fi < nf) {
+          const float2 gg = a.gout[((long long)(f0 + fi) * a.gout_tx + gt) * a.n_pix + p];
+          // conj(w e^{j 2 pi p}) * g
+          const float br = fmaf(gg.x, wr, gg.y * wi);
+          const float bi = fmaf(gg.y, wr, -gg.x * wi);
+          float* dst = acc + 2 * (fi * ns + g.m - LO);
+#pragma unroll
+          for (int k = 0; k < NT; ++k) {
+            atomicAdd(dst + 2 * k, q[k] * br);
+            atomicAdd(dst + 2 * k + 1, q[k] * bi);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int fi = 0; fi < nf; ++fi) {
+    float* gdst = reinterpret_cast<float*>(
+        a.gin + (((long long)(f0 + fi) * a.n_tx + t) * a.n_ch + c) * ns);
+    const float* src = acc + 2 * fi * ns;
+    for (int i = tid; i < 2 * ns; i += kThreads) {
+      if (a.atomic_flush) unsafeAtomicAdd(gdst + i, src[i]);
+      else gdst[i] = a.accumulate ? gdst[i] + src[i] : src[i];
+    }
+  }
+}
+
+__global__ void fill_zero(float* __restrict__ p, long long n) {
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long long i = i0; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+
+// ----------------------------------------------------------------------------
+// Host side.
+// ----------------------------------------------------------------------------
+int g_tune_v = 0, g_tune_fb = 0, g_tune_split = 0;
+
+int check_desc(const rca_das_desc* d) {
+  if (!d) return fail(RCA_ERR_INVALID, "desc is NULL");
+  if (d->n_pix < 0 || d->n_channel < 1 || d->n_tx < 1 || d->n_sample < 1)
+    return fail(RCA_ERR_INVALID, "bad sizes: nPix=%d nChannel=%d nTx=%d nSample=%d", d->n_pix,
+                d->n_channel, d->n_tx, d->n_sample);
+  if (d->n_pix > 0 && (!d->grid_x || !d->grid_y || !d->grid_z))
+    return fail(RCA_ERR_INVALID, "grid pointers are NULL");
+  if (!d->ele_x || !d->ele_y || !d->ele_z) return fail(RCA_ERR_INVALID, "element pointers are NULL");
+  if (!d->vs_x || !d->vs_y || !d->vs_z || !d->min_dist)
+    return fail(RCA_ERR_INVALID, "virtual-source pointers are NULL");
+  if (d->interp < RCA_INTERP_QUADRATIC || d->interp > RCA_INTERP_LANCZOS4)
+    return fail(RCA_ERR_INVALID, "unknown interpolation %d", d->interp);
+  if (d->ab_law_mode < 0 || d->ab_law_mode > 2)
+    return fail(RCA_ERR_INVALID, "unknown aberration-law mode %d", d->ab_law_mode);
+  if (d->ab_law_mode && !d->ab_law_rx) return fail(RCA_ERR_INVALID, "ab_law_rx is NULL");
+  if (!(d->speed_of_sound > 0.0f) || !(d->sampling_frequency > 0.0f))
+    return fail(RCA_ERR_INVALID, "speed of sound and sampling frequency must be > 0");
+  if (!(d->fnumber != 0.0f)) return fail(RCA_ERR_INVALID, "fnumber must be non-zero");
+  return RCA_OK;
+}
+
+KArgs make_args(const rca_das_desc* d, int n_frame) {
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.gx = d->grid_x; a.gy = d->grid_y; a.gz = d->grid_z;
+  a.ex = d->ele_x; a.ey = d->ele_y; a.ez = d->ele_z;
+  a.vx = d->vs_x; a.vy = d->vs_y; a.vz = d->vs_z; a.d0 = d->min_dist;
+  a.ab = d->ab_law_rx; a.ab_mode = d->ab_law_rx ? d->ab_law_mode : 0;
+  const double fs = d->sampling_frequency, fd = d->demodulation_frequency, c = d->speed_of_sound;
+  a.kfs = fs / c; a.kfd = fd / c;
+  a.ut0 = fs * (double)d->start_time; a.pt0 = fd * (double)d->start_time;
+  a.fs = fs; a.fd = fd;
+  a.apx = d->aperture_x; a.apy = d->aperture_y; a.apz = d->aperture_z; a.fnum = d->fnumber;
+  a.n_pix = d->n_pix; a.n_ch = d->n_channel; a.n_tx = d->n_tx; a.n_sample = d->n_sample;
+  a.n_frame = n_frame;
+  a.receive_x = d->receive_x != 0; a.mask_tx = d->mask_transmit != 0;
+  return a;
+}
+
+template <typename S> struct SName;
+
+template <int KIND, int V, int FB, typename S>
+int launch_fwd_t(const KArgs& a, hipStream_t st) {
+  const size_t lds = (size_t)FB * a.n_sample * sizeof(S);
+  const unsigned gx = (unsigned)(((long long)a.n_pix + kThreads * V - 1) / (kThreads * V));
+  const unsigned gy = (unsigned)((a.n_frame + FB - 1) / FB);
+  if (gx == 0 || gy == 0) return RCA_OK;
+  hipLaunchKernelGGL((das_fwd<KIND, V, FB, S>), dim3(gx, gy, 1), dim3(kThreads), lds, st, a);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RCA_OK : hip_fail(e, "das_fwd launch");
+}
+
+template <int KIND, typename S>
+int launch_fwd_k(const KArgs& a, int V, int FB, hipStream_t st) {
+#define RCA_FWD_CASE(VV, FF) \
+  if (V == VV && FB == FF) return launch_fwd_t<KIND, VV, FF, S>(a, st);
+  RCA_FWD_CASE(1, 1) RCA_FWD_CASE(2, 1) RCA_FWD_CASE(4, 1)
+  RCA_FWD_CASE(1, 2) RCA_FWD_CASE(2, 2) RCA_FWD_CASE(4, 2)
+  RCA_FWD_CASE(1, 4) RCA_FWD_CASE(2, 4) RCA_FWD_CASE(4, 4)
+  RCA_FWD_CASE(1, 8) RCA_FWD_CASE(2, 8)
+#undef RCA_FWD_CASE
+  return fail(RCA_ERR_UNSUPPORTED, "no kernel for V=%d FB=%d", V, FB);
+}
+
+template <typename S>
+int launch_fwd_s(const KArgs& a, int interp, int V, int FB, hipStream_t st) {
+  switch (interp) {
+    case RCA_INTERP_QUADRATIC: return launch_fwd_k<RCA_INTERP_QUADRATIC, S>(a, V, FB, st);
+    case RCA_INTERP_LANCZOS_REF: return launch_fwd_k<RCA_INTERP_LANCZOS_REF, S>(a, V, FB, st);
+    case RCA_INTERP_LANCZOS3: return launch_fwd_k<RCA_INTERP_LANCZOS3, S>(a, V, FB, st);
+    case RCA_INTERP_LANCZOS4: return launch_fwd_k<RCA_INTERP_LANCZOS4, S>(a, V, FB, st);
+  }
+  return fail(RCA_ERR_INVALID, "unknown interpolation %d", interp);
+}
+
+int pick_fb(int n_frame, int n_sample, size_t bytes) {
+  int fb = 8;
+  while (fb > 1 && (fb > n_frame || (size_t)fb * n_sample * bytes > (size_t)kMaxLds)) fb >>= 1;
+  return fb;
+}
+
+int pick_v(long long n_pix, int n_fblocks, int fb) {
+  // Enough blocks to give every CU (256 on MI355X) several workgroups.
+  const long long target = 4096;
+  for (int v : {4, 2}) {
+    if (fb == 8 && v == 4) continue;
+    const long long blocks = ((n_pix + kThreads * v - 1) / (kThreads * v)) * n_fblocks;
+    if (blocks >= target) return v;
+  }
+  return 1;
+}
+
+template <int KIND, int V, int FB>
+int launch_adj_t(const KArgs& a, hipStream_t st) {
+  const size_t lds = (size_t)FB * a.n_sample * 2 * sizeof(float);
+  const dim3 grid(a.n_ch, a.n_tx, a.n_split * ((a.n_frame + FB - 1) / FB));
+  hipLaunchKernelGGL((das_adj<KIND, V, FB>), grid, dim3(kThreads), lds, st, a);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RCA_OK : hip_fail(e, "das_adj launch");
+}
+
+template <int KIND>
+int launch_adj_k(const KArgs& a, int FB, hipStream_t st) {
+  switch (FB) {
+    case 1: return launch_adj_t<KIND, 2, 1>(a, st);
+    case 2: return launch_adj_t<KIND, 2, 2>(a, st);
+    case 4: return launch_adj_t<KIND, 2, 4>(a, st);
+    case 8: return launch_adj_t<KIND, 2, 8>(a, st);
+  }
+  return fail(RCA_ERR_UNSUPPORTED, "no adjoint kernel for FB=%d", FB);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rca_last_error(void) { return g_err.c_str(); }
+
+const char* rca_version(void) { return "rca_das 0.1.0 gfx950"; }
+
+// Tuning override for sweeps: workgroup voxels per lane (V), frames per block
+// (FB) and adjoint voxel splits; 0 restores the automatic choice.
+int rca_set_tuning(int v, int fb, int split) {
+  if ((v != 0 && v != 1 && v != 2 && v != 4) || (fb != 0 && fb != 1 && fb != 2 && fb != 4 && fb != 8) ||
+      split < 0)
+    return fail(RCA_ERR_INVALID, "bad tuning V=%d FB=%d split=%d", v, fb, split);
+  g_tune_v = v;
+  g_tune_fb = fb;
+  g_tune_split = split;
+  return RCA_OK;
+}
+
+int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float* trace_scale,
+                int n_frame, void* out, float* mask, int flags, hipStream_t stream) {
+  g_err.clear();
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (n_frame < 0) return fail(RCA_ERR_INVALID, "nFrame < 0");
+  if (!iq || !out) return fail(RCA_ERR_INVALID, "iq/out pointers are NULL");
+  if (iq_dtype != RCA_IQ_C64 && iq_dtype != RCA_IQ_F16X2)
+    return fail(RCA_ERR_INVALID, "unknown iq dtype %d", iq_dtype);
+  if (iq_dtype == RCA_IQ_F16X2 && !trace_scale)
+    return fail(RCA_ERR_INVALID, "trace_scale is required for FP16 I/Q");
+  if (flags & ~(RCA_FLAG_COMPOUND | RCA_FLAG_ACCUMULATE))
+    return fail(RCA_ERR_INVALID, "unknown flags 0x%x", flags);
+  if (n_frame == 0 || d->n_pix == 0) return RCA_OK;
+  const size_t sbytes = iq_dtype == RCA_IQ_C64 ? 8 : 4;
+  if ((size_t)d->n_sample * sbytes > (size_t)kMaxLds)
+    return fail(RCA_ERR_UNSUPPORTED, "nSample=%d exceeds the LDS trace capacity (%d bytes)",
+                d->n_sample, kMaxLds);
+  if (d->ab_law_mode == RCA_ABLAW_TX_CHANNEL && !d->ab_law_rx)
+    return fail(RCA_ERR_INVALID, "ab_law_rx is NULL");
+  KArgs a = make_args(d, n_frame);
+  a.iq = iq;
+  a.scale = trace_scale;
+  a.out = reinterpret_cast<float2*>(out);
+  a.mask = mask;
+  a.compound = (flags & RCA_FLAG_COMPOUND) != 0;
+  a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
+  a.vec_ok = (((uintptr_t)iq & 15) == 0) && (((size_t)d->n_sample * sbytes) % 16 == 0);
+  int fb = g_tune_fb ? g_tune_fb : pick_fb(n_frame, d->n_sample, sbytes);
+  if ((size_t)fb * d->n_sample * sbytes > (size_t)kMaxLds)
+    return fail(RCA_ERR_UNSUPPORTED, "FB=%d does not fit in LDS", fb);
+  const int nfb = (n_frame + fb - 1) / fb;
+  int v = g_tune_v ? g_tune_v : pick_v(d->n_pix, nfb, fb);
+  if (fb == 8 && v == 4) v = 2;
+  if (iq_dtype == RCA_IQ_C64) return launch_fwd_s<float2>(a, d->interp, v, fb, stream);
+  return launch_fwd_s<__half2>(a, d->interp, v, fb, stream);
+}
+
+int rca_adjoint(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_frame,
+                void* grad_iq, int flags, hipStream_t stream) {
+  g_err.clear();
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!grad_out || !grad_iq) return fail(RCA_ERR_INVALID, "grad pointers are NULL");
+  if (grad_tx != 1 && grad_tx != d->n_tx)
+    return fail(RCA_ERR_INVALID, "grad_tx must be 1 or nTx (got %d)", grad_tx);
+  if (n_frame < 0) return fail(RCA_ERR_INVALID, "nFrame < 0");
+  if (flags & ~RCA_FLAG_ACCUMULATE) return fail(RCA_ERR_INVALID, "unknown flags 0x%x", flags);
+  if (n_frame == 0) return RCA_OK;
+  if ((size_t)d->n_sample * 8 > (size_t)kMaxLds)
+    return fail(RCA_ERR_UNSUPPORTED, "nSample=%d exceeds the LDS trace capacity", d->n_sample);
+  KArgs a = make_args(d, n_frame);
+  a.gout = reinterpret_cast<const float2*>(grad_out);
+  a.gin = reinterpret_cast<float2*>(grad_iq);
+  a.gout_tx = grad_tx;
+  a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
+  const int fb = pick_fb(n_frame, d->n_sample, 8);
+  const int nfb = (n_frame + fb - 1) / fb;
+  const long long traces = (long long)d->n_channel * d->n_tx * nfb;
+  int split = g_tune_split;
+  if (!split) {
+    split = (int)((2048 + traces - 1) / traces);
+    const long long maxs = ((long long)d->n_pix + kThreads * 8 - 1) / (kThreads * 8);
+    if (split > maxs) split = (int)(maxs > 0 ? maxs : 1);
+    if (split < 1) split = 1;
+  }
+  a.n_split = split;
+  a.atomic_flush = split > 1;
+  if (a.atomic_flush && !a.accumulate) {
+    const long long n = 2LL * n_frame * d->n_tx * d->n_channel * d->n_sample;
+    hipLaunchKernelGGL(fill_zero, dim3(2048), dim3(256), 0, stream,
+                       reinterpret_cast<float*>(grad_iq), n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "fill_zero launch");
+  }
+  if (d->n_pix == 0) {
+    if (!a.atomic_flush && !a.accumulate) {
+      const long long n = 2LL * n_frame * d->n_tx * d->n_channel * d->n_sample;
+      hipLaunchKernelGGL(fill_zero, dim3(2048), dim3(256), 0, stream,
+                         reinterpret_cast<float*>(grad_iq), n);
+    }
+    return RCA_OK;
+  }
+  switch (d->interp) {
+    case RCA_INTERP_QUADRATIC: return launch_adj_k<RCA_INTERP_QUADRATIC>(a, fb, stream);
+    case RCA_INTERP_LANCZOS_REF: return launch_adj_k<RCA_INTERP_LANCZOS_REF>(a, fb, stream);
+    case RCA_INTERP_LANCZOS3: return launch_adj_k<RCA_INTERP_LANCZOS3>(a, fb, stream);
+    case RCA_INTERP_LANCZOS4: return launch_adj_k<RCA_INTERP_LANCZOS4>(a, fb, stream);
+  }
+  return fail(RCA_ERR_INVALID, "unknown interpolation %d", d->interp);
+}
+
+int bfVirtualSourceGeneric_RCA(float* dataBf, float* mask, const float* data,
+                               const float* gridBfX, const float* gridBfY, const float* gridBfZ,
+                               const float* posEleX, const float* posEleY, const float* posEleZ,
+                               const float* virtualSourceX, const float* virtualSourceY,
+                               const float* virtualSourceZ, const float* minDistVirtualSource,
+                               int flagReceiveX, float apertureX, float apertureY, float apertureZ,
+                               float samplingFrequency, float demodulationFrequency,
+                               float speedOfSound, float startTime, float fnumber, int nPix,
+                               int nChannel, int nTx, int nFrame, int nSample,
+                               int flagInterpolationLanczos, int flagMaskTransmit,
+                               hipStream_t stream) {
+  rca_das_desc d;
+  memset(&d, 0, sizeof(d));
+  d.grid_x = gridBfX; d.grid_y = gridBfY; d.grid_z = gridBfZ;
+  d.ele_x = posEleX; d.ele_y = posEleY; d.ele_z = posEleZ;
+  d.vs_x = virtualSourceX; d.vs_y = virtualSourceY; d.vs_z = virtualSourceZ;
+  d.min_dist = minDistVirtualSource;
+  d.n_pix = nPix; d.n_channel = nChannel; d.n_tx = nTx; d.n_sample = nSample;
+  d.receive_x = flagReceiveX; d.mask_transmit = flagMaskTransmit;
+  d.interp = flagInterpolationLanczos ? RCA_INTERP_LANCZOS_REF : RCA_INTERP_QUADRATIC;
+  d.aperture_x = apertureX; d.aperture_y = apertureY; d.aperture_z = apertureZ;
+  d.sampling_frequency = samplingFrequency; d.demodulation_frequency = demodulationFrequency;
+  d.speed_of_sound = speedOfSound; d.start_time = startTime; d.fnumber = fnumber;
+  if (!dataBf || !mask) return fail(RCA_ERR_INVALID, "dataBf/mask pointers are NULL");
+  return rca_forward(&d, data, RCA_IQ_C64, nullptr, nFrame, dataBf, mask, 0, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,268 @@
+"""Device-resident beamforming plan of one RCA half, over the C ABI.
+
+``RcaPlan`` keeps the voxel grid, centred element positions, virtual sources
+and minimal distances on the GPU (the reference re-uploads them on every
+``feval``, bfVirtualSourceGeneric_RCA.m:670-686) and launches the HIP kernels
+on torch's current stream.  torch provides device memory and streams only; all
+beamforming arithmetic is in ``csrc/rca_das.hip``.
+
+Tensor layouts (C-order):
+  iq     complex64 [nFrame, nTx, nChannel, nSample], or the FP16-I/Q form
+         (float16 [nFrame, nTx, nChannel, nSample, 2], float32 scale
+         [nFrame, nTx, nChannel]) made by :func:`quantize_fp16`
+  out    complex64 [nFrame, 1 | nTx, nPix]
+  mask   float32   [1 | nTx, nPix]
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _np64(a):
+    """Column-major flatten to float64 numpy (host prep runs on the CPU)."""
+    if isinstance(a, torch.Tensor):
+        a = a.detach().cpu()
+        a = a.permute(*reversed(range(a.dim()))).reshape(-1) if a.dim() > 1 else a.reshape(-1)
+        return a.to(torch.float64).numpy()
+    return np.asarray(a, dtype=np.float64).ravel(order="F")
+
+
+def prepare_geometry(Geometry: dict, Transmit: dict) -> dict:
+    """Host prep of bfVirtualSourceGeneric_RCA.m:491-558 in float64, cast to
+    float32 as the wrapper does before the launch (RCA.m:601-617).
+
+    * element positions are centred on their mean (RCA.m:500-502);
+    * missing apertures default to the span of the centred positions
+      (RCA.m:505-521);
+    * tilts become virtual sources at r = 10 m (RCA.m:527-541);
+    * minDist_t = min_c |(ey_c - vx_t, ex_c - vy_t, ez_c - vz_t)| — the X/Y
+      swap is the reference's (RCA.m:551-558).
+    """
+    pos = [_np64(Geometry[k]) for k in ("posEleX", "posEleY", "posEleZ")]
+    cen = [p - p.mean() for p in pos]
+    ap = []
+    for k, c in zip(("apertureX", "apertureY", "apertureZ"), cen):
+        ap.append(float(Geometry[k]) if k in Geometry else float(c.max() - c.min()))
+    if "thetaX" in Transmit:
+        tx = np.tan(_np64(Transmit["thetaX"]))
+        ty = np.tan(_np64(Transmit["thetaY"]))
+        vz = -10.0 / np.sqrt(1.0 + tx ** 2 + ty ** 2)
+        vx, vy = vz * tx, vz * ty
+    else:
+        vx, vy, vz = (_np64(Transmit[k])
+                      for k in ("virtualSourceX", "virtualSourceY", "virtualSourceZ"))
+    ex, ey, ez = cen
+    d0 = np.sqrt((ey[None, :] - vx[:, None]) ** 2 + (ex[None, :] - vy[:, None]) ** 2 +
+                 (ez[None, :] - vz[:, None]) ** 2).min(axis=1)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    return {"ele": [f32(c) for c in cen], "vs": [f32(vx), f32(vy), f32(vz)], "d0": f32(d0),
+            "aperture": [float(np.float32(a)) for a in ap]}
+
+
+def _dev_f32(a, device):
+    if isinstance(a, torch.Tensor):
+        t = a.detach().to(device=device, dtype=torch.float32)
+        return t.reshape(-1) if t.is_contiguous() else t.contiguous().reshape(-1)
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel(order="F"))).to(device)
+
+
+def quantize_fp16(iq: torch.Tensor):
+    """FP16-I/Q storage: half2 samples scaled by a per-trace FP32 factor.
+
+    scale[f, t, c] = max over the trace of max(|re|, |im|) (1 for an all-zero
+    trace); samples are stored as (re, im) / scale in float16.  Returns
+    (float16 [F, T, C, Ns, 2], float32 [F, T, C]).
+    """
+    r = torch.view_as_real(iq)
+    scale = r.abs().amax(dim=(-1, -2)).to(torch.float32)
+    scale = torch.where(scale > 0, scale, torch.ones_like(scale))
+    h = (r / scale[..., None, None]).to(torch.float16).contiguous()
+    return h, scale.contiguous()
+
+
+def dequantize_fp16(h: torch.Tensor, scale: torch.Tensor) -> torch.Tensor:
+    return torch.view_as_complex((h.to(torch.float32) * scale[..., None, None]).contiguous())
+
+
+class RcaPlan:
+    """Geometry of one RCA half resident on a GPU.
+
+    Arguments follow the reference kernel's (bfVirtualSourceGeneric_RCA.cu:103-133):
+    grids are flat voxel lists (column-major flatten of the meshgrid), element
+    positions are already centred, virtual sources and minimal distances are
+    per tilt.  ``interp`` is 'quadratic', 'lanczos' (the reference's 6-tap
+    kernel), 'lanczos3' or 'lanczos4'.
+    """
+
+    def __init__(self, *, grid, ele, vs, min_dist, receive_x, aperture, fs, fd, c, t0, fnumber,
+                 n_sample, interp="quadratic", mask_transmit=False, ab_law=None,
+                 ab_law_mode=None, device=None):
+        self.lib = _lib.load()
+        self.device = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        if self.device.type != "cuda":
+            raise ValueError("RcaPlan needs a GPU device")
+        dev = self.device
+        self.grid = [_dev_f32(g, dev) for g in grid]
+        self.ele = [_dev_f32(e, dev) for e in ele]
+        self.vs = [_dev_f32(v, dev) for v in vs]
+        self.min_dist = _dev_f32(min_dist, dev)
+        n_pix = self.grid[0].numel()
+        n_ch = self.ele[0].numel()
+        n_tx = self.vs[0].numel()
+        if any(g.numel() != n_pix for g in self.grid):
+            raise ValueError("gridBfX/Y/Z must have the same number of voxels")
+        if any(e.numel() != n_ch for e in self.ele):
+            raise ValueError("posEleX/Y/Z must have the same length")
+        if any(v.numel() != n_tx for v in self.vs) or self.min_dist.numel() != n_tx:
+            raise ValueError("virtual sources and minDist must have one entry per tilt")
+        if interp not in _lib.INTERP:
+            raise ValueError(f"unknown interpolation {interp!r}")
+        self.ab_law = None
+        mode = 0
+        if ab_law is not None:
+            self.ab_law = _dev_f32(ab_law, dev)
+            if ab_law_mode is None:
+                ab_law_mode = "channel" if self.ab_law.numel() == n_ch else "tx_channel"
+            mode = _lib.ABLAW[ab_law_mode]
+            need = n_ch if mode == 1 else n_tx * n_ch
+            if self.ab_law.numel() != need:
+                raise ValueError(f"aberration law needs {need} delays, got {self.ab_law.numel()}")
+        self.n_pix, self.n_channel, self.n_tx, self.n_sample = n_pix, n_ch, n_tx, int(n_sample)
+        self.interp = interp
+        self.receive_x = bool(receive_x)
+        d = _lib.RcaDasDesc()
+        d.grid_x, d.grid_y, d.grid_z = (g.data_ptr() for g in self.grid)
+        d.ele_x, d.ele_y, d.ele_z = (e.data_ptr() for e in self.ele)
+        d.vs_x, d.vs_y, d.vs_z = (v.data_ptr() for v in self.vs)
+        d.min_dist = self.min_dist.data_ptr()
+        d.ab_law_rx = self.ab_law.data_ptr() if self.ab_law is not None else None
+        d.ab_law_mode = mode
+        d.n_pix, d.n_channel, d.n_tx, d.n_sample = n_pix, n_ch, n_tx, self.n_sample
+        d.receive_x = int(self.receive_x)
+        d.mask_transmit = int(bool(mask_transmit))
+        d.interp = _lib.INTERP[interp]
+        d.aperture_x, d.aperture_y, d.aperture_z = (float(a) for a in aperture)
+        d.sampling_frequency = float(fs)
+        d.demodulation_frequency = float(fd)
+        d.speed_of_sound = float(c)
+        d.start_time = float(t0)
+        d.fnumber = float(fnumber)
+        self.desc = d
+
+    # ------------------------------------------------------------------
+    @classmethod
+    def from_reference(cls, Geometry, Transmit, Param, Options=None, n_sample=None, device=None):
+        """Plan from the reference's Geometry/Transmit/Param/Options dicts."""
+        Options = dict(Options or {})
+        prep = prepare_geometry(Geometry, Transmit)
+        grid = [np.asarray(Geometry[k], dtype=np.float32) if not isinstance(Geometry[k], torch.Tensor)
+                else Geometry[k] for k in ("gridBfX", "gridBfY", "gridBfZ")]
+        return cls(grid=grid, ele=prep["ele"], vs=prep["vs"], min_dist=prep["d0"],
+                   receive_x=bool(Transmit.get("flagReceiveX", True)), aperture=prep["aperture"],
+                   fs=Param["samplingFrequency"], fd=Param["demodulationFrequency"],
+                   c=Param.get("speedOfSound", 1540.0), t0=Param.get("startTime", 0.0),
+                   fnumber=Param.get("fnumber", 1.0), n_sample=n_sample,
+                   interp=interpolation_name(Options),
+                   mask_transmit=bool(Options.get("flagMaskTransmit", True)),
+                   ab_law=Param.get("AbLawRx"), ab_law_mode=Param.get("AbLawMode"), device=device)
+
+    # ------------------------------------------------------------------
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _check_iq(self, iq, scale):
+        if iq.device != self.device:
+            raise ValueError(f"iq is on {iq.device}, plan on {self.device}")
+        if not iq.is_contiguous():
+            raise ValueError("iq must be contiguous")
+        if iq.dtype == torch.complex64:
+            if iq.dim() != 4:
+                raise ValueError("complex iq must be [nFrame, nTx, nChannel, nSample]")
+            dtype = _lib.IQ_C64
+        elif iq.dtype == torch.float16:
+            if iq.dim() != 5 or iq.shape[-1] != 2:
+                raise ValueError("FP16 iq must be [nFrame, nTx, nChannel, nSample, 2]")
+            if scale is None or scale.dtype != torch.float32 or not scale.is_contiguous() \
+                    or tuple(scale.shape) != tuple(iq.shape[:3]) or scale.device != self.device:
+                raise ValueError("FP16 iq needs a contiguous float32 scale [nFrame, nTx, nChannel]")
+            dtype = _lib.IQ_F16X2
+        else:
+            raise ValueError(f"unsupported iq dtype {iq.dtype}")
+        f, t, c, n = iq.shape[:4]
+        if (t, c, n) != (self.n_tx, self.n_channel, self.n_sample):
+            raise ValueError(f"iq shape {tuple(iq.shape)} does not match the plan "
+                             f"(nTx={self.n_tx}, nChannel={self.n_channel}, nSample={self.n_sample})")
+        return dtype, f
+
+    def forward(self, iq, scale=None, *, compound=True, out=None, mask=None, want_mask=True,
+                accumulate=False):
+        """Beamform ``iq``; returns (out [F, 1|T, P] complex64, mask [1|T, P] or None)."""
+        dtype, nf = self._check_iq(iq, scale)
+        nt = 1 if compound else self.n_tx
+        if out is None:
+            if accumulate:
+                raise ValueError("accumulate needs an output tensor")
+            out = torch.empty((nf, nt, self.n_pix), dtype=torch.complex64, device=self.device)
+        elif (out.dtype != torch.complex64 or not out.is_contiguous() or out.device != self.device
+              or tuple(out.shape) != (nf, nt, self.n_pix)):
+            raise ValueError(f"out must be contiguous complex64 {(nf, nt, self.n_pix)} on {self.device}")
+        if want_mask and mask is None:
+            if accumulate:
+                raise ValueError("accumulate with a mask needs the mask tensor")
+            mask = torch.empty((nt, self.n_pix), dtype=torch.float32, device=self.device)
+        if mask is not None and (mask.dtype != torch.float32 or not mask.is_contiguous()
+                                 or tuple(mask.shape) != (nt, self.n_pix) or mask.device != self.device):
+            raise ValueError(f"mask must be contiguous float32 {(nt, self.n_pix)}")
+        flags = (_lib.FLAG_COMPOUND if compound else 0) | (_lib.FLAG_ACCUMULATE if accumulate else 0)
+        rc = self.lib.rca_forward(ctypes.byref(self.desc), ctypes.c_void_p(iq.data_ptr()), dtype,
+                                  ctypes.c_void_p(scale.data_ptr() if scale is not None else 0),
+                                  nf, ctypes.c_void_p(out.data_ptr()),
+                                  ctypes.c_void_p(mask.data_ptr() if mask is not None else 0),
+                                  flags, self._stream())
+        _lib.check(rc)
+        return out, mask
+
+    def adjoint(self, grad_out, *, grad_iq=None, accumulate=False):
+        """grad_iq = A^H grad_out for grad_out complex64 [F, 1|T, P]."""
+        if (grad_out.dtype != torch.complex64 or grad_out.dim() != 3 or not grad_out.is_contiguous()
+                or grad_out.shape[2] != self.n_pix or grad_out.shape[1] not in (1, self.n_tx)
+                or grad_out.device != self.device):
+            raise ValueError("grad_out must be contiguous complex64 [nFrame, 1|nTx, nPix] on the plan device")
+        nf, gt = grad_out.shape[0], grad_out.shape[1]
+        shape = (nf, self.n_tx, self.n_channel, self.n_sample)
+        if grad_iq is None:
+            if accumulate:
+                raise ValueError("accumulate needs grad_iq")
+            grad_iq = torch.empty(shape, dtype=torch.complex64, device=self.device)
+        elif (grad_iq.dtype != torch.complex64 or tuple(grad_iq.shape) != shape
+              or not grad_iq.is_contiguous() or grad_iq.device != self.device):
+            raise ValueError(f"grad_iq must be contiguous complex64 {shape}")
+        rc = self.lib.rca_adjoint(ctypes.byref(self.desc), ctypes.c_void_p(grad_out.data_ptr()), gt,
+                                  nf, ctypes.c_void_p(grad_iq.data_ptr()),
+                                  _lib.FLAG_ACCUMULATE if accumulate else 0, self._stream())
+        _lib.check(rc)
+        return grad_iq
+
+    def updates(self, n_frame: int = 1) -> int:
+        """Dense voxel*tilt*element updates of ``n_frame`` frames."""
+        return self.n_pix * self.n_tx * self.n_channel * n_frame
+
+
+def interpolation_name(Options: dict) -> str:
+    """Options -> kernel interpolation.  ``flagInterpolationLanczos`` keeps the
+    reference meaning (RCA.m:159-169); ``interpolation`` selects
+    'quadratic' | 'lanczos' | 'lanczos3' | 'lanczos4' explicitly."""
+    kind = Options.get("interpolation")
+    if kind is None:
+        return "lanczos" if Options.get("flagInterpolationLanczos", False) else "quadratic"
+    kind = str(kind).lower()
+    if kind not in _lib.INTERP:
+        raise ValueError(f"unknown interpolation {kind!r}")
+    return kind

@@ -1,0 +1,96 @@
+"""GPU parity of the FP16-I/Q forward and of the adjoint kernel.
+
+FP16-I/Q bar: relative L2 <= 3e-3 and envelope error <= 0.1 dB against the
+FP64 oracle on the original complex64 data; <= 1e-5 against the FP64 oracle
+on the dequantised data (the kernel's arithmetic is FP32).
+Adjoint bar: relative L2 <= 1e-5 against the FP64 oracle adjoint; the
+dot-product identity <A x, y> = <x, A^H y> to 1e-5.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import rca_oracle as O
+from simu_rca_ab_amd import _lib, configs as C
+from simu_rca_ab_amd.plan import RcaPlan, dequantize_fp16, quantize_fp16
+from tests import helpers as H
+from tests.test_gpu_forward import check, oracle_fwd, plan_for, say
+
+pytestmark = pytest.mark.gpu
+
+CFG = C.scaled(C.get("cfg1"), nx=6, ny=6, nz=48, n_scat=300)
+
+
+@pytest.mark.parametrize("half_index", [0, 1])
+def test_fp16_iq(dev, half_index):
+    say(f"case FP16-I/Q half={half_index}, 3 frames, lanczos4")
+    iq = H.iq_frame(CFG, half_index, n_frame=3)
+    plan, G, T = plan_for(CFG, half_index, dev)
+    h, s = quantize_fp16(iq.to(dev))
+    out16, m16 = plan.forward(h, s)
+    out32, _ = plan.forward(iq.to(dev))
+    torch.cuda.synchronize()
+    ref, _ = oracle_fwd(CFG, half_index, iq, "lanczos4")
+    ref = ref.sum(axis=1, keepdims=True)
+    check(out16.cpu().numpy(), ref, "fp16 vs FP64 oracle (original data)", rel=3e-3)
+    check(out16.cpu().numpy(), out32.cpu().numpy(), "fp16 vs FP32 kernel", rel=3e-3)
+    deq = dequantize_fp16(h, s).cpu()
+    refq, _ = oracle_fwd(CFG, half_index, deq, "lanczos4")
+    check(out16.cpu().numpy(), refq.sum(axis=1, keepdims=True), "fp16 vs FP64 oracle (dequantised)")
+
+
+ADJ = C.scaled(C.get("cfg0"), nx=8, ny=6, nz=24)
+
+
+@pytest.mark.parametrize("interp", ["lanczos3", "quadratic"])
+@pytest.mark.parametrize("gtx", [1, 11])
+def test_adjoint_vs_oracle(dev, interp, gtx):
+    say(f"case adjoint interp={interp} grad_tx={gtx}")
+    plan, G, T = plan_for(ADJ, 0, dev, interp)
+    rng = np.random.default_rng(1)
+    g = (rng.standard_normal((2, gtx, plan.n_pix)) + 1j * rng.standard_normal((2, gtx, plan.n_pix))
+         ).astype(np.complex64)
+    got = plan.adjoint(torch.from_numpy(g).to(dev))
+    torch.cuda.synchronize()
+    prep = O.prepare(G, T)
+    kind, a = {"lanczos3": (O.INTERP_LANCZOS, 3), "quadratic": (O.INTERP_QUADRATIC, 3)}[interp]
+    ref = O.kernel_adjoint(g, H.grid_f32(G), prep, H.oracle_params(ADJ, ADJ.halves[0]), 2, kind, a)
+    r = H.rel_l2(got.cpu().numpy(), ref)
+    say(f"adjoint rel-L2 {r:.3e}")
+    assert r <= 1e-5
+
+
+def test_adjoint_split_and_accumulate(dev):
+    say("case adjoint voxel splits (atomic flush) and accumulate")
+    plan, G, T = plan_for(ADJ, 0, dev, "lanczos3")
+    rng = np.random.default_rng(2)
+    g = torch.from_numpy((rng.standard_normal((3, 1, plan.n_pix)) +
+                          1j * rng.standard_normal((3, 1, plan.n_pix))).astype(np.complex64)).to(dev)
+    try:
+        _lib.set_tuning(0, 0, 1)
+        a = plan.adjoint(g)
+        _lib.set_tuning(0, 0, 5)
+        b = plan.adjoint(g)
+        c = b.clone()
+        plan.adjoint(g, grad_iq=c, accumulate=True)
+    finally:
+        _lib.set_tuning(0, 0, 0)
+    torch.cuda.synchronize()
+    assert H.rel_l2(b.cpu().numpy(), a.cpu().numpy()) <= 1e-6
+    assert H.rel_l2(c.cpu().numpy(), 2 * a.cpu().numpy()) <= 1e-6
+
+
+def test_dot_product_identity(dev):
+    say("case <A x, y> = <x, A^H y> on the GPU")
+    plan, G, T = plan_for(ADJ, 0, dev, "lanczos3")
+    x = H.iq_frame(ADJ, 0, n_frame=2).to(dev)
+    rng = np.random.default_rng(4)
+    y = torch.from_numpy((rng.standard_normal((2, 1, plan.n_pix)) +
+                          1j * rng.standard_normal((2, 1, plan.n_pix))).astype(np.complex64)).to(dev)
+    Ax, _ = plan.forward(x)
+    AHy = plan.adjoint(y)
+    lhs = torch.vdot(y.flatten().to(torch.complex128), Ax.flatten().to(torch.complex128))
+    rhs = torch.vdot(AHy.flatten().to(torch.complex128), x.flatten().to(torch.complex128))
+    r = abs(complex(lhs - rhs)) / abs(complex(lhs))
+    say(f"dot-product mismatch {r:.3e}")
+    assert r <= 1e-5
