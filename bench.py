@@ -1,0 +1,222 @@
+"""Benchmark of the RCA delay-and-sum beamformer on one MI355X.
+
+Workload (one "step"): one full RCA frame of the 128+128-element probe —
+2 x 16 tilts (-12..12 deg) of 128-line traces with 2048 complex64 samples,
+beamformed on 128x128x256 voxels with Lanczos a=4, the two halves summed in
+place (two launches of the forward kernel).  Inputs are seeded synthetic I/Q
+already resident in HBM when the timed region starts.
+
+Metric: voxel*tilt*element updates per second, counted densely
+(nVox * nTilt * nElem per frame).  Prints one JSON line.
+
+Extra legs:
+  * roofline of the dominant kernel (the forward DAS launch): algorithmic bytes
+    = updates per launch x taps x 8 B (the interpolation footprint), divided by
+    the launch's mean duration from HIP events on its stream;
+  * cpu_baseline: the FP32 CPU restatement of the reference kernel (OpenMP on
+    every host core) on a bounded voxel sample of the same workload;
+  * fp16: the FP16-I/Q mode on a 16-frame ensemble of the same geometry.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_plans(cfg, dev):
+    from simu_rca_ab_amd import configs as C
+    from simu_rca_ab_amd.plan import RcaPlan
+    grid = C.meshgrid_zxy(*C.grid_axes(cfg))
+    plans = []
+    for h in cfg.halves:
+        G = C.geometry(cfg, h, grid)
+        for k in ("gridBfX", "gridBfY", "gridBfZ"):
+            G[k] = G[k].ravel(order="F")
+        plans.append(RcaPlan.from_reference(G, C.transmit(h), C.param(cfg), C.options(cfg),
+                                            n_sample=cfg.n_sample, device=dev))
+    return plans
+
+
+def run_gpu(cfg, steps, warmup, dev):
+    import torch
+    from simu_rca_ab_amd import synth
+    log(f"[bench] synthesising {cfg.name} I/Q on {dev}")
+    iqs = [x.to(dev).contiguous() for x in synth.frame(cfg, device=dev)]
+    plans = build_plans(cfg, dev)
+    out = torch.empty((1, 1, plans[0].n_pix), dtype=torch.complex64, device=dev)
+    mask = torch.empty((1, plans[0].n_pix), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    ev = []
+
+    def step(record):
+        for i, (p, iq) in enumerate(zip(plans, iqs)):
+            if record:
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+            p.forward(iq, out=out, mask=mask, accumulate=i > 0)
+            if record:
+                b.record(stream)
+                ev.append((a, b))
+
+    log(f"[bench] warmup {warmup}")
+    for _ in range(warmup):
+        step(False)
+    torch.cuda.synchronize()
+    log(f"[bench] timing {steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    return dt, launch_ms, plans, iqs, out
+
+
+def run_fp16(cfg, dev, n_frame=16, reps=3):
+    """FP16-I/Q ensemble throughput on one half-pair of the same geometry."""
+    import torch
+    from simu_rca_ab_amd import synth
+    from simu_rca_ab_amd.plan import quantize_fp16
+    plans = build_plans(cfg, dev)
+    iqs = [x.to(dev) for x in synth.frame(cfg, device=dev)]
+    # an ensemble of n_frame frames (frame content varies by a per-frame phase)
+    res = {}
+    for name, make in (("fp16", True), ("fp32", False)):
+        data = []
+        for iq in iqs:
+            ph = torch.exp(1j * torch.linspace(0, 3, n_frame, device=dev)).to(torch.complex64)
+            ens = (iq * ph[:, None, None, None]).contiguous()
+            data.append(quantize_fp16(ens) if make else (ens, None))
+        out = torch.empty((n_frame, 1, plans[0].n_pix), dtype=torch.complex64, device=dev)
+        for _ in range(1):
+            for i, (p, (x, s)) in enumerate(zip(plans, data)):
+                p.forward(x, s, out=out, want_mask=False, accumulate=i > 0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for i, (p, (x, s)) in enumerate(zip(plans, data)):
+                p.forward(x, s, out=out, want_mask=False, accumulate=i > 0)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[name] = cfg.updates_per_frame() * n_frame / dt
+        del data, out
+    return res
+
+
+def cpu_baseline(cfg, target_s=6.0):
+    """FP32 CPU restatement of the reference kernel on a voxel sample of the
+    workload, OpenMP over every core this process may use."""
+    import numpy as np
+    from oracle import rca_oracle as O
+    from simu_rca_ab_amd import configs as C, synth
+    cores = len(os.sched_getaffinity(0))
+    iqs = synth.frame(cfg)
+    grid = C.meshgrid_zxy(*C.grid_axes(cfg))
+    flat = [g.ravel(order="F").astype(np.float32) for g in grid]
+
+    def run(nvox):
+        t = 0.0
+        sel = np.linspace(0, cfg.n_vox - 1, nvox).astype(np.int64)
+        gs = tuple(np.ascontiguousarray(f[sel]) for f in flat)
+        for hi, h in enumerate(cfg.halves):
+            G, T = C.geometry(cfg, h, grid), C.transmit(h)
+            prep = O.prepare(G, T)
+            params = dict(receive_x=h.receive_x, fs=cfg.fs, fd=cfg.fd, c=cfg.c, t0=cfg.t0,
+                          fnumber=cfg.fnumber, n_sample=cfg.n_sample)
+            d = iqs[hi].numpy().view(np.float32)
+            t1 = time.perf_counter()
+            O.kernel_forward(d, gs, prep, params, O.INTERP_LANCZOS, 4, precision=32,
+                             nthreads=cores)
+            t += time.perf_counter() - t1
+        return t
+
+    probe = 2048
+    tp = run(probe)
+    nvox = int(min(cfg.n_vox, max(probe, probe * target_s / max(tp, 1e-3) / 3)))
+    times = [run(nvox) for _ in range(3)]
+    t = statistics.median(times)
+    upd = nvox * cfg.n_tilt * cfg.n_elem
+    return {"value": upd / t, "unit": "voxel*tilt*element updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{nvox} of {cfg.n_vox} voxels (evenly spaced), all 32 tilts x 128 elements, "
+                      f"1 frame; FP32 CPU restatement of bfVirtualSourceGeneric_RCA.cu, OpenMP "
+                      f"{cores} threads, median of 3 runs ({statistics.median(times):.2f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-I/Q leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X; "
+              "multi-GPU sharding (frames/voxels across GPUs) is designed, not built.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from simu_rca_ab_amd import configs as C
+    cfg = C.get("cfg1")
+    dev = torch.device("cuda", 0)
+    dt, launch_ms, plans, iqs, out = run_gpu(cfg, args.steps, args.warmup, dev)
+    ms_step = dt * 1e3 / args.steps
+    upd_step = cfg.updates_per_frame()
+    value = upd_step / (dt / args.steps)
+    mean_launch = statistics.mean(launch_ms)
+    upd_launch = upd_step / len(cfg.halves)
+    bytes_launch = upd_launch * cfg.taps() * 8
+    achieved = bytes_launch / (mean_launch * 1e-3) / 1e9
+    log(f"[bench] {value / 1e9:.2f} G updates/s, {ms_step:.2f} ms/step, "
+        f"launch mean {mean_launch:.2f} ms")
+    del plans, iqs, out
+    torch.cuda.empty_cache()
+    extra = {}
+    if not args.no_fp16:
+        log("[bench] FP16-I/Q ensemble leg (16 frames)")
+        r = run_fp16(cfg, dev)
+        extra["fp16_iq"] = {"value": r["fp16"], "fp32_same_ensemble": r["fp32"],
+                            "unit": "voxel*tilt*element updates/s", "frames": 16,
+                            "vs_plain_fp32_single_frame": r["fp16"] / value}
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline leg")
+        cpu = cpu_baseline(cfg)
+        cpu["gpu_speedup"] = value / cpu["value"]
+    line = {
+        "metric": "voxel*tilt*element updates/s (FP32 drop-in, 1 frame)",
+        "value": value, "unit": "voxel*tilt*element updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded analytic point scatterers + 20 dB noise)",
+        "config": {"workload": "cfg1: RCA 128 row + 128 column elements, 2 x 16 tilts -12..12 deg, "
+                               "2048-sample complex64 I/Q, 128x128x256 voxels, Lanczos a=4, 1 frame",
+                   "voxels": cfg.n_vox, "tilts": cfg.n_tilt, "elements": cfg.n_elem,
+                   "samples": cfg.n_sample, "interp": cfg.interpolation, "frames": 1},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "das_fwd<lanczos4>", "launch_ms": mean_launch,
+                     "bytes_per_update": cfg.taps() * 8, "updates_per_launch": upd_launch},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
