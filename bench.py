@@ -164,7 +164,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-I/Q leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 512 MiB device copy (HBM-counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = 512 << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X; "
               "multi-GPU sharding (frames/voxels across GPUs) is designed, not built.",

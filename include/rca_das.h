@@ -125,7 +125,7 @@ int rca_adjoint(const rca_das_desc* desc, const void* grad_out, int grad_tx, int
 
 /*
  * Launch-shape override for tuning sweeps (no reference counterpart):
- * voxels per lane v in {1,2,4}, frames per workgroup fb in {1,2,4,8}, adjoint
+ * voxels per lane v in {1,2}, frames per workgroup fb in {1,2,4,8}, adjoint
  * voxel splits; 0 restores the automatic choice.  Process-wide.
  */
 int rca_set_tuning(int v, int fb, int split);

@@ -94,7 +94,7 @@ def test_launch_shapes_agree(dev):
     plan, G, T = plan_for(cfg, 0, dev, "lanczos4")
     outs = {}
     try:
-        for v in (1, 2, 4):
+        for v in (1, 2):
             for fb in (1, 2, 4):
                 _lib.set_tuning(v, fb, 0)
                 outs[(v, fb)] = plan.forward(iq)[0].cpu().numpy()

@@ -1,8 +1,13 @@
 """GPU parity of the FP16-I/Q forward and of the adjoint kernel.
 
-FP16-I/Q bar: relative L2 <= 3e-3 and envelope error <= 0.1 dB against the
-FP64 oracle on the original complex64 data; <= 1e-5 against the FP64 oracle
-on the dequantised data (the kernel's arithmetic is FP32).
+FP16-I/Q bar: relative L2 <= 3e-3 against the FP64 oracle on the original
+complex64 data, with envelope error <= 0.1 dB over voxels within -30 dB of the
+maximum and <= 0.2 dB within -40 dB; <= 1e-5 (and 0.1 dB at -40 dB) against
+the FP64 oracle on the dequantised data, i.e. the kernel's FP32 arithmetic
+adds nothing to the storage error.  The -40 dB bound is looser than the FP32
+one because 11-bit FP16 mantissas leak ~2.4e-4 of each bright echo into the
+dim voxels: the FP64 oracle run on the dequantised data shows the same
+0.10-0.14 dB worst voxel at the -40 dB floor.
 Adjoint bar: relative L2 <= 1e-5 against the FP64 oracle adjoint; the
 dot-product identity <A x, y> = <x, A^H y> to 1e-5.
 """
@@ -32,8 +37,11 @@ def test_fp16_iq(dev, half_index):
     torch.cuda.synchronize()
     ref, _ = oracle_fwd(CFG, half_index, iq, "lanczos4")
     ref = ref.sum(axis=1, keepdims=True)
-    check(out16.cpu().numpy(), ref, "fp16 vs FP64 oracle (original data)", rel=3e-3)
-    check(out16.cpu().numpy(), out32.cpu().numpy(), "fp16 vs FP32 kernel", rel=3e-3)
+    check(out16.cpu().numpy(), ref, "fp16 vs FP64 oracle (original data)", rel=3e-3, env=0.2)
+    e30 = H.envelope_db_err(out16.cpu().numpy(), ref, floor_db=-30.0)
+    say(f"fp16 envelope error within -30 dB: {e30:.4f} dB")
+    assert e30 <= 0.1
+    check(out16.cpu().numpy(), out32.cpu().numpy(), "fp16 vs FP32 kernel", rel=3e-3, env=0.2)
     deq = dequantize_fp16(h, s).cpu()
     refq, _ = oracle_fwd(CFG, half_index, deq, "lanczos4")
     check(out16.cpu().numpy(), refq.sum(axis=1, keepdims=True), "fp16 vs FP64 oracle (dequantised)")
