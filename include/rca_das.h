@@ -12,7 +12,8 @@
  *            float scale [nFrame][nTx][nChannel]           (RCA_IQ_F16X2)
  *            == the reference's data [2*nSample, nChannel, nTx, nFrame]
  *            column-major buffer (bfVirtualSourceGeneric_RCA.m:576-578).
- *   out      complex64 [nFrame][1 | nTx][nPix]  (1 when RCA_FLAG_COMPOUND)
+ *   out      complex64 [nFrame][1 | nTx][nPix]  (1 when RCA_FLAG_COMPOUND),
+ *            or [nChannel][nFrame][nTx][nPix] with RCA_FLAG_NOSUM
  *            == the reference's dataBf [2*nPix, nTx|1, nFrame]
  *            (bfVirtualSourceGeneric_RCA.cu:230, .m:654-658).
  *   mask     float [1 | nTx][nPix] (sum of Hann weights, .cu:301-304).
@@ -45,7 +46,9 @@ typedef enum { RCA_IQ_C64 = 0, RCA_IQ_F16X2 = 1 } rca_iq_dtype;
 
 enum {
   RCA_FLAG_COMPOUND = 1,   /* sum over tilts in the kernel (.m:690-697) */
-  RCA_FLAG_ACCUMULATE = 2  /* out += result (adds an RCA half, script_All.m:382) */
+  RCA_FLAG_ACCUMULATE = 2, /* out += result (adds an RCA half, script_All.m:382) */
+  RCA_FLAG_NOSUM = 4       /* one output per receive line, out [nChannel][nFrame][nTx][nPix]
+                              (NoSum/bfVirtualSourceGeneric_RCA_Nosum.cu:228-229, 376-380) */
 };
 
 typedef enum { RCA_ABLAW_NONE = 0, RCA_ABLAW_CHANNEL = 1, RCA_ABLAW_TX_CHANNEL = 2 } rca_ablaw;
@@ -125,7 +128,7 @@ int rca_adjoint(const rca_das_desc* desc, const void* grad_out, int grad_tx, int
 
 /*
  * Launch-shape override for tuning sweeps (no reference counterpart):
- * voxels per lane v in {1,2}, frames per workgroup fb in {1,2,4,8}, adjoint
+ * voxels per lane v in {1,2,4}, frames per workgroup fb in {1,2,4,8}, adjoint
  * voxel splits; 0 restores the automatic choice.  Process-wide.
  */
 int rca_set_tuning(int v, int fb, int split);

@@ -15,7 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "librca_das.so"
 RCA_OK, RCA_ERR_INVALID, RCA_ERR_UNSUPPORTED, RCA_ERR_HIP, RCA_ERR_BUDGET = range(5)
 INTERP = {"quadratic": 0, "lanczos": 1, "lanczos3": 2, "lanczos4": 3}
 IQ_C64, IQ_F16X2 = 0, 1
-FLAG_COMPOUND, FLAG_ACCUMULATE = 1, 2
+FLAG_COMPOUND, FLAG_ACCUMULATE, FLAG_NOSUM = 1, 2, 4
 ABLAW = {None: 0, "channel": 1, "tx_channel": 2}
 
 # Every symbol include/rca_das.h declares.

@@ -21,7 +21,9 @@ import torch
 
 from ..plan import RcaPlan, interpolation_name, prepare_geometry
 
-__all__ = ["bfVirtualSourceGeneric_RCA", "RcaInputError", "RcaWarning"]
+__all__ = ["bfVirtualSourceGeneric_RCA", "bfVirtualSourceGeneric_RCA_Nosum",
+           "bfVirtualSourceGeneric_RCA_ab_1", "bfVirtualSourceGeneric_RCA_ab_2",
+           "bfVirtualSourceGeneric_RCA_Nosum_ab", "RcaInputError", "RcaWarning"]
 
 _ID = "bfVirtualSourceGeneric_RCA"
 
@@ -246,6 +248,35 @@ def bfVirtualSourceGeneric_RCA(*args, device=None):
     (RCA.m:715-722).  With no argument it builds the HIP library (the
     reference compiles its PTX in that case, RCA.m:112-115).
     """
+    return _run(args, device, nosum=False, ablaw=None)
+
+
+def bfVirtualSourceGeneric_RCA_Nosum(*args, device=None):
+    """Per-receive-line variant (MATLAB/lib/beamformer/NoSum/bfVirtualSourceGeneric_RCA_Nosum.m):
+    no channel sum and no compounding; ``dataBf`` is complex64
+    [s1, s2, s3, nTx, nFrame, nChannel] (Nosum.m:653-718), ``mask`` per tilt."""
+    return _run(args, device, nosum=True, ablaw=None)
+
+
+def bfVirtualSourceGeneric_RCA_ab_1(*args, device=None):
+    """Receive aberration law per line, ``Param.AbLawRx`` [nChannel] (s) added to
+    the receive delay (Hyp1_save/bfVirtualSourceGeneric_RCA_ab_1.cu:306-311)."""
+    return _run(args, device, nosum=False, ablaw="channel")
+
+
+def bfVirtualSourceGeneric_RCA_ab_2(*args, device=None):
+    """Receive aberration law per (tilt, line), ``Param.AbLawRx`` with
+    AbLawRx(:)[tx*nChannel + ch] (Hyp2/bfVirtualSourceGeneric_RCA_ab_2.cu:306-311)."""
+    return _run(args, device, nosum=False, ablaw="tx_channel")
+
+
+def bfVirtualSourceGeneric_RCA_Nosum_ab(*args, device=None):
+    """NoSum with a per-line receive aberration law
+    (NoSum_ab/bfVirtualSourceGeneric_RCA_Nosum_ab.cu:308-313)."""
+    return _run(args, device, nosum=True, ablaw="channel")
+
+
+def _run(args, device, nosum, ablaw):
     if len(args) == 0:
         from .. import build
         build.build_library()
@@ -255,6 +286,8 @@ def bfVirtualSourceGeneric_RCA(*args, device=None):
     data, Geometry, Transmit, Param = args[:4]
     Options = args[4] if len(args) == 5 else None
     Transmit, Param, Options = _validate(data, Geometry, Transmit, Param, Options)
+    if ablaw is not None and "AbLawRx" not in Param:
+        raise RcaInputError("AbLawRxMissing", "Receive aberration law ('Param.AbLawRx') is required!")
 
     as_torch = isinstance(data, torch.Tensor)
     if device is None:
@@ -281,24 +314,34 @@ def bfVirtualSourceGeneric_RCA(*args, device=None):
         Geo[k] = flat_grid(Geometry[k])
     Par = {k: _scalar(Param[k]) for k in ("samplingFrequency", "demodulationFrequency",
                                           "speedOfSound", "startTime", "fnumber")}
-    for k in ("AbLawRx", "AbLawMode"):
-        if k in Param:
-            Par[k] = Param[k]
-    Tr = dict(Transmit)
-    plan = RcaPlan.from_reference(Geo, Tr, Par, Options, n_sample=ns_iq, device=device)
+    if ablaw is not None:
+        law = Param["AbLawRx"]
+        law = (law.detach().cpu().numpy() if isinstance(law, torch.Tensor) else np.asarray(law))
+        Par["AbLawRx"] = law.astype(np.float32).ravel(order="F")   # single(Param.AbLawRx)
+        Par["AbLawMode"] = ablaw
+    plan = RcaPlan.from_reference(Geo, dict(Transmit), Par, Options, n_sample=ns_iq, device=device)
 
-    compound = bool(Options["flagCompounding"])
-    out, mask = plan.forward(iq, compound=compound)
-    nf, nt = out.shape[0], out.shape[1]
+    compound = bool(Options["flagCompounding"]) and not nosum
+    out, mask = plan.forward(iq, compound=compound, nosum=nosum)
     s1, s2, s3 = grid_shape
+    if nosum:
+        nc, nf, nt = out.shape[0], out.shape[1], out.shape[2]
+        if as_torch:
+            dataBf = out.reshape(nc, nf, nt, s3, s2, s1).permute(5, 4, 3, 2, 1, 0)
+        else:
+            dataBf = out.cpu().numpy().transpose(3, 2, 1, 0).reshape(s1, s2, s3, nt, nf, nc,
+                                                                     order="F")
+    else:
+        nf, nt = out.shape[0], out.shape[1]
+        if as_torch:
+            dataBf = out.reshape(nf, nt, s3, s2, s1).permute(4, 3, 2, 1, 0)
+        else:
+            dataBf = out.cpu().numpy().transpose(2, 1, 0).reshape(s1, s2, s3, nt, nf, order="F")
+    nt = mask.shape[0]
     if as_torch:
-        dataBf = out.reshape(nf, nt, s3, s2, s1).permute(4, 3, 2, 1, 0)
         maskr = mask.reshape(nt, s3, s2, s1).permute(3, 2, 1, 0)
         if not data.is_cuda:
             dataBf, maskr = dataBf.cpu(), maskr.cpu()
         return dataBf, maskr
-    o = out.cpu().numpy()
-    m = mask.cpu().numpy()
-    dataBf = o.transpose(2, 1, 0).reshape(s1, s2, s3, nt, nf, order="F")
-    maskr = m.transpose(1, 0).reshape(s1, s2, s3, nt, order="F")
+    maskr = mask.cpu().numpy().transpose(1, 0).reshape(s1, s2, s3, nt, order="F")
     return dataBf, maskr

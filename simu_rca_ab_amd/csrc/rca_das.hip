@@ -77,8 +77,8 @@ struct KArgs {
   double fs, fd;
   float apx, apy, apz, fnum;
   int n_pix, n_ch, n_tx, n_sample, n_frame;
-  int receive_x, mask_tx, ab_mode, compound, accumulate, vec_ok;
-  int gout_tx, n_split, atomic_flush, piped;
+  int receive_x, mask_tx, ab_mode, compound, accumulate, vec_ok, nosum;
+  int gout_tx, n_split, atomic_flush;
 };
 
 // ----------------------------------------------------------------------------
@@ -327,31 +327,23 @@ __device__ __forceinline__ void ab_terms(const KArgs& a, int t, int c, double& u
 // ----------------------------------------------------------------------------
 // Forward kernel.
 //
-// Loop nest of one workgroup (256 lanes x V voxels, FB frames):
-//   for each block of TB tilts:
-//     per (voxel, tilt): transmit sample offset Bu (FP64) and the transmit
-//       phasor exp(j 2 pi frac(Bp)) — once per tilt block
-//     for each receive line c:
-//       per (voxel, line): receive distance R (FP64), Hann weight, receive
-//         sample offset kfs*R and w * exp(j 2 pi frac(kfd*R)) — once for the
-//         TB tilts of the block
-//       for each tilt t of the block:   (trace (t, c) staged in LDS)
-//         per update: u = Bu + kfs*R (FP64), floor/fraction, 2a weights,
-//         2a LDS taps, accT[t] += taps * w * exp(j 2 pi frac(kfd*R))
-//     per (voxel, tilt): out += accT[t] * transmit phasor
-// The demodulation phase is exp(j 2 pi (p_tx + p_rx)): the transmit factor
-// does not depend on the line, so it multiplies the per-tilt line sum once.
-// Traces are double-buffered in LDS with the next one prefetched into
-// registers while the current one is used.
+// One workgroup: 256 lanes x V consecutive voxels x FB frames.  Loop nest
+// tilt -> receive line; for every (tilt, line) the FB traces are staged in
+// LDS once and each lane evaluates its V updates from them.  Per update: FP64
+// receive distance, FP64 sample position and phase (range-reduced before the
+// FP32 conversion), FP32 Hann weight, interpolation weights, 2a LDS taps and a
+// complex FMA into the per-tilt partial sum.  Per-tilt partial sums are added
+// to the compounded result after each tilt (two-level summation).
+//
+// Measured on MI355X this simple nest at V=2 (143 VGPRs, 3 waves/SIMD) beats
+// variants that reuse the receive terms across tilts at the cost of more
+// registers (2 waves/SIMD).
 // ----------------------------------------------------------------------------
-constexpr int kTB = 4;   // tilts per block
-constexpr int kPF = 4;   // 16-byte prefetch chunks per lane (16 KB per stage)
-static_assert(kPF == 4, "the prefetch registers r0..r3 assume four chunks");
-
 template <int KIND, int V, int FB, typename S>
 __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  S* tr = reinterpret_cast<S*>(smem_raw);
   const S* iq = reinterpret_cast<const S*>(a.iq);
 
   const int tid = threadIdx.x;
@@ -360,29 +352,6 @@ __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
   const int nf = min(FB, a.n_frame - f0);
   const int ns = a.n_sample;
   const long long fstride = (long long)a.n_tx * a.n_ch * ns;
-  const double hiLim = (double)(ns - 1 - HI);
-
-  // Staging: the FB traces of one (tilt, line) are nf*n16 chunks of 16 bytes;
-  // lane tid moves chunks tid + k*256.  Pipelined when they fit in kPF
-  // registers per lane (a.vec_ok also guarantees 16-byte alignment).
-  const int n16 = (int)(((long long)ns * sizeof(S)) / 16);
-  const int nchunk = nf * n16;
-  const bool piped = a.piped;   // host: vec_ok && FB * n16 <= kPF * kThreads
-  S* const buf0 = reinterpret_cast<S*>(smem_raw);
-  const int buf1_off = piped ? FB * ns : 0;   // second LDS buffer (pipelined path)
-  // four 16-byte chunks per lane (kPF == 4), held in scalars so they stay in registers
-  int g0, g1, g2, g3;
-  {
-    auto off = [&](int i) -> int {
-      const int fi = n16 ? i / n16 : 0;
-      return i < nchunk ? (int)(fi * (fstride * (long long)sizeof(S) / 16) + (i - fi * n16)) : -1;
-    };
-    g0 = off(tid);
-    g1 = off(tid + kThreads);
-    g2 = off(tid + 2 * kThreads);
-    g3 = off(tid + 3 * kThreads);
-  }
-  uint4 r0 = make_uint4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
 
   double px[V], py[V], pz[V], apF[V], iapF[V];
   bool valid[V];
@@ -403,199 +372,125 @@ __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
     for (int fi = 0; fi < FB; ++fi) acc[v][fi] = make_float2(0.0f, 0.0f);
   }
 
-  for (int tb0 = 0; tb0 < a.n_tx; tb0 += kTB) {
-    const int nt = min(kTB, a.n_tx - tb0);
-    double Bu[V][kTB];
-    float2 ptx[V][kTB];
-    bool act[V][kTB];
-    float2 accT[V][kTB][FB];
-    float mW[V];
+  for (int t = 0; t < a.n_tx; ++t) {
+    const float vxf = a.vx[t], vyf = a.vy[t], vzf = a.vz[t];
+    const double d0 = a.d0[t];
+    double Bu[V], Bp[V];
+    bool act[V];
+    float2 accT[V][FB];
+    float mT[V];
 #pragma unroll
-    for (int t = 0; t < kTB; ++t) {
-      const int tt = tb0 + (t < nt ? t : 0);
-      const float vxf = a.vx[tt], vyf = a.vy[tt], vzf = a.vz[tt];
-      const double d0 = a.d0[tt];
+    for (int v = 0; v < V; ++v) {
+      tx_terms(a, px[v], py[v], pz[v], vxf, vyf, vzf, d0, Bu[v], Bp[v]);
+      act[v] = valid[v] && (!a.mask_tx || tx_mask(a, vxf, vyf, vzf, (float)px[v], (float)py[v],
+                                                   (float)pz[v]));
+      mT[v] = 0.0f;
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        double Bp;
-        tx_terms(a, px[v], py[v], pz[v], vxf, vyf, vzf, d0, Bu[v][t], Bp);
-        float sn, cs;
-        sincos2pi((float)(Bp - rint(Bp)), sn, cs);
-        ptx[v][t] = make_float2(cs, sn);
-        act[v][t] = t < nt && valid[v] &&
-                    (!a.mask_tx || tx_mask(a, vxf, vyf, vzf, (float)px[v], (float)py[v], (float)pz[v]));
-#pragma unroll
-        for (int fi = 0; fi < FB; ++fi) accT[v][t][fi] = make_float2(0.0f, 0.0f);
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < V; ++v) mW[v] = 0.0f;
-
-    const int nstep = nt * a.n_ch;
-    // trace (tb0 + step % nt, step / nt) of frame f0, in samples
-#define RCA_TRACE_BASE(step) \
-  (((long long)f0 * a.n_tx * a.n_ch + (long long)(tb0 + (step) % nt) * a.n_ch + (step) / nt) * ns)
-#define RCA_LOAD(step)                                                             \
-  {                                                                                \
-    const uint4* src_ = reinterpret_cast<const uint4*>(iq + RCA_TRACE_BASE(step)); \
-    if (g0 >= 0) r0 = src_[g0];                                                    \
-    if (g1 >= 0) r1 = src_[g1];                                                    \
-    if (g2 >= 0) r2 = src_[g2];                                                    \
-    if (g3 >= 0) r3 = src_[g3];                                                    \
-  }
-#define RCA_STORE(dst)                                                             \
-  {                                                                                \
-    uint4* d_ = reinterpret_cast<uint4*>(dst);                                     \
-    if (g0 >= 0) d_[tid] = r0;                                                     \
-    if (g1 >= 0) d_[tid + kThreads] = r1;                                          \
-    if (g2 >= 0) d_[tid + 2 * kThreads] = r2;                                      \
-    if (g3 >= 0) d_[tid + 3 * kThreads] = r3;                                      \
-  }
-    if (piped) {
-      __syncthreads();
-      RCA_LOAD(0);
-      RCA_STORE(buf0);
-      __syncthreads();
+      for (int fi = 0; fi < FB; ++fi) accT[v][fi] = make_float2(0.0f, 0.0f);
     }
 
-    // receive-line terms of the current line
-    double urx[V];
-    float2 wph[V];
-    bool inap[V];
-    for (int step = 0; step < nstep; ++step) {
-      const int c = step / nt, t = step % nt;
-      S* tr = buf0 + ((step & 1) ? buf1_off : 0);
-      if (piped) {
-        if (step + 1 < nstep) RCA_LOAD(step + 1);
-      } else {
-        __syncthreads();
-        stage<S>(buf0, iq + RCA_TRACE_BASE(step), fstride, nf, ns, a.vec_ok);
-        __syncthreads();
-        tr = buf0;
-      }
-      if (t == 0) {
-        const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
-        double uab = 0.0, pab = 0.0;
-        if (a.ab_mode == RCA_ABLAW_CHANNEL) {
-          const double ab = a.ab[c];
-          uab = a.fs * ab;
-          pab = a.fd * ab;
-        }
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          const double A = a.receive_x ? fabs(px[v] - exd) : fabs(py[v] - eyd);   // .cu:284-291
-          inap[v] = A < apF[v];                                                  // .cu:294
-          const float w = inap[v] ? hann((float)(A * iapF[v])) : 0.0f;           // .cu:299
-          mW[v] += w;                                                            // .cu:301-304
-          const double dz = pz[v] - ezd;
-          const double R = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));            // .cu:308-309
-          urx[v] = fma(a.kfs, R, uab);
-          const double prx = fma(a.kfd, R, pab);
-          float sn, cs;
-          sincos2pi((float)(prx - rint(prx)), sn, cs);
-          wph[v] = make_float2(w * cs, w * sn);
-        }
-      }
-      float2 rot = make_float2(1.0f, 0.0f);
-      if (a.ab_mode == RCA_ABLAW_TX_CHANNEL) {
-        const double ab = a.ab[(tb0 + t) * a.n_ch + c];
-        const double pab = a.fd * ab;
-        float sn, cs;
-        sincos2pi((float)(pab - rint(pab)), sn, cs);
-        rot = make_float2(cs, sn);
-      }
-      const double uab_t = a.ab_mode == RCA_ABLAW_TX_CHANNEL ? a.fs * a.ab[(tb0 + t) * a.n_ch + c] : 0.0;
+    for (int c = 0; c < a.n_ch; ++c) {
+      __syncthreads();
+      stage<S>(tr, iq + ((long long)f0 * a.n_tx * a.n_ch + (long long)t * a.n_ch + c) * ns,
+               fstride, nf, ns, a.vec_ok);
+      __syncthreads();
+      const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
+      double uab, pab;
+      ab_terms(a, t, c, uab, pab);
       float sc[FB];
 #pragma unroll
       for (int fi = 0; fi < FB; ++fi) {
         sc[fi] = 1.0f;
         if constexpr (sizeof(S) == 4) {
-          if (fi < nf) sc[fi] = a.scale[((long long)(f0 + fi) * a.n_tx + tb0 + t) * a.n_ch + c];
+          if (fi < nf) sc[fi] = a.scale[((long long)(f0 + fi) * a.n_tx + t) * a.n_ch + c];
         }
+      }
+      if (a.nosum) {   // per-line output: the partial sum restarts at every line
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+          for (int fi = 0; fi < FB; ++fi) accT[v][fi] = make_float2(0.0f, 0.0f);
       }
 #pragma unroll
       for (int v = 0; v < V; ++v) {
+        if (!act[v]) continue;
+        const Upd g = update_geom<LO, HI>(a, px[v], py[v], pz[v], apF[v], iapF[v], Bu[v], Bp[v],
+                                          exd, eyd, ezd, uab, pab);
+        mT[v] += g.w;   // .cu:301-304 (inside the aperture, before the range test)
+        if (!g.in_rng) continue;
+        float q[NT];
+        interp_weights<KIND>(g.d, q);
+        float sn, cs;
+        sincos2pi(g.p, sn, cs);
+        const float wr = g.w * cs, wi = g.w * sn;
+        const int base = g.m - LO;
 #pragma unroll
-        for (int tk = 0; tk < kTB; ++tk) {
-          if (tk != t) continue;
-          if (!(act[v][tk] && inap[v])) continue;
-          const double u = Bu[v][tk] + urx[v] + uab_t;                             // .cu:310-311
-          const double mf = floor(u);                                            // .cu:319
-          if (!(mf >= (double)LO && mf < hiLim)) continue;                       // .cu:323
-          const int m = (int)mf;
-          float q[NT];
-          interp_weights<KIND>((float)(u - mf), q);
-          float2 wv = wph[v];
-          if (a.ab_mode == RCA_ABLAW_TX_CHANNEL)
-            wv = make_float2(wv.x * rot.x - wv.y * rot.y, wv.x * rot.y + wv.y * rot.x);
-          const S* trb = tr + (m - LO);
+        for (int fi = 0; fi < FB; ++fi) {
+          if (fi < nf) {
+            const S* trf = tr + fi * ns + base;
+            float re = 0.0f, im = 0.0f;
 #pragma unroll
-          for (int fi = 0; fi < FB; ++fi) {
-            if (fi < nf) {
-              const S* trf = trb + fi * ns;
-              float re = 0.0f, im = 0.0f;
-#pragma unroll
-              for (int k = 0; k < NT; ++k) {
-                const float2 s = ld_sample(trf, k);
-                re = fmaf(q[k], s.x, re);
-                im = fmaf(q[k], s.y, im);
-              }
-              const float r2 = wv.x * sc[fi], i2 = wv.y * sc[fi];
-              accT[v][tk][fi].x = fmaf(re, r2, fmaf(-im, i2, accT[v][tk][fi].x));
-              accT[v][tk][fi].y = fmaf(re, i2, fmaf(im, r2, accT[v][tk][fi].y));
+            for (int k = 0; k < NT; ++k) {
+              const float2 s = ld_sample(trf, k);
+              re = fmaf(q[k], s.x, re);
+              im = fmaf(q[k], s.y, im);
             }
+            const float r2 = wr * sc[fi], i2 = wi * sc[fi];
+            accT[v][fi].x = fmaf(re, r2, fmaf(-im, i2, accT[v][fi].x));
+            accT[v][fi].y = fmaf(re, i2, fmaf(im, r2, accT[v][fi].y));
           }
         }
       }
-      if (piped) {
-        if (step + 1 < nstep) RCA_STORE(buf0 + (((step + 1) & 1) ? buf1_off : 0));
-        __syncthreads();
+      if (a.nosum) {   // NoSum.cu:228-229,275-278,376-380: out[c][f][t][p]
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          if (!valid[v]) continue;
+          const long long p = pblk + v * kThreads + tid;
+#pragma unroll
+          for (int fi = 0; fi < FB; ++fi) {
+            if (fi < nf) {
+              float2* o = a.out + (((long long)c * a.n_frame + f0 + fi) * a.n_tx + t) * a.n_pix + p;
+              float2 r = accT[v][fi];
+              if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
+              *o = r;
+            }
+          }
+        }
       }
     }
 
-    // per (voxel, tilt): apply the transmit phasor, then compound or write
+    if (a.compound) {
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const long long p = pblk + v * kThreads + tid;
-#pragma unroll
-      for (int t = 0; t < kTB; ++t) {
-        if (t >= nt) continue;
-        const float mt = act[v][t] ? mW[v] : 0.0f;
-        float2 r[FB];
+      for (int v = 0; v < V; ++v) {
+        macc[v] += mT[v];
 #pragma unroll
         for (int fi = 0; fi < FB; ++fi) {
-          const float2 x = accT[v][t][fi], e = ptx[v][t];
-          r[fi] = make_float2(x.x * e.x - x.y * e.y, x.x * e.y + x.y * e.x);
+          acc[v][fi].x += accT[v][fi].x;
+          acc[v][fi].y += accT[v][fi].y;
         }
-        if (a.compound) {
-          macc[v] += mt;
+      }
+    } else {
 #pragma unroll
-          for (int fi = 0; fi < FB; ++fi) {
-            acc[v][fi].x += r[fi].x;
-            acc[v][fi].y += r[fi].y;
-          }
-        } else if (valid[v]) {
+      for (int v = 0; v < V; ++v) {
+        if (!valid[v]) continue;
+        const long long p = pblk + v * kThreads + tid;
 #pragma unroll
-          for (int fi = 0; fi < FB; ++fi) {
-            if (fi < nf) {
-              float2* o = a.out + ((long long)(f0 + fi) * a.n_tx + tb0 + t) * a.n_pix + p;
-              float2 y = r[fi];
-              if (a.accumulate) { const float2 old = *o; y.x += old.x; y.y += old.y; }
-              *o = y;
-            }
+        for (int fi = 0; fi < FB; ++fi) {
+          if (fi < nf && !a.nosum) {
+            float2* o = a.out + ((long long)(f0 + fi) * a.n_tx + t) * a.n_pix + p;
+            float2 r = accT[v][fi];
+            if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
+            *o = r;
           }
-          if (a.mask && blockIdx.y == 0) {
-            float* mk = a.mask + (long long)(tb0 + t) * a.n_pix + p;
-            *mk = a.accumulate ? *mk + mt : mt;
-          }
+        }
+        if (a.mask && blockIdx.y == 0) {
+          float* mk = a.mask + (long long)t * a.n_pix + p;
+          *mk = a.accumulate ? *mk + mT[v] : mT[v];
         }
       }
     }
   }
 
-#undef RCA_TRACE_BASE
-#undef RCA_LOAD
-#undef RCA_STORE
   if (a.compound) {
 #pragma unroll
     for (int v = 0; v < V; ++v) {
@@ -746,10 +641,7 @@ template <typename S> struct SName;
 
 template <int KIND, int V, int FB, typename S>
 int launch_fwd_t(KArgs a, hipStream_t st) {
-  const long long n16 = (long long)a.n_sample * sizeof(S) / 16;
-  const long long fstride16 = (long long)a.n_tx * a.n_ch * a.n_sample * sizeof(S) / 16;
-  a.piped = a.vec_ok && FB * n16 <= (long long)kPF * kThreads && FB * fstride16 < (1LL << 31);
-  const size_t lds = (size_t)(a.piped ? 2 : 1) * FB * a.n_sample * sizeof(S);
+  const size_t lds = (size_t)FB * a.n_sample * sizeof(S);
   const unsigned gx = (unsigned)(((long long)a.n_pix + kThreads * V - 1) / (kThreads * V));
   const unsigned gy = (unsigned)((a.n_frame + FB - 1) / FB);
   if (gx == 0 || gy == 0) return RCA_OK;
@@ -762,8 +654,9 @@ template <int KIND, typename S>
 int launch_fwd_k(const KArgs& a, int V, int FB, hipStream_t st) {
 #define RCA_FWD_CASE(VV, FF) \
   if (V == VV && FB == FF) return launch_fwd_t<KIND, VV, FF, S>(a, st);
-  RCA_FWD_CASE(1, 1) RCA_FWD_CASE(2, 1) RCA_FWD_CASE(1, 2) RCA_FWD_CASE(2, 2)
-  RCA_FWD_CASE(1, 4) RCA_FWD_CASE(1, 8)
+  RCA_FWD_CASE(1, 1) RCA_FWD_CASE(2, 1) RCA_FWD_CASE(4, 1)
+  RCA_FWD_CASE(1, 2) RCA_FWD_CASE(2, 2) RCA_FWD_CASE(1, 4) RCA_FWD_CASE(2, 4)
+  RCA_FWD_CASE(1, 8)
 #undef RCA_FWD_CASE
   return fail(RCA_ERR_UNSUPPORTED, "no kernel for V=%d FB=%d", V, FB);
 }
@@ -786,15 +679,10 @@ int pick_fb(int n_frame, int n_sample, size_t bytes) {
 }
 
 int pick_v(long long n_pix, int n_fblocks, int fb) {
-  // Enough blocks to give every CU (256 on MI355X) several workgroups; the
-  // register budget allows V * FB <= 4 (V = 1 at FB >= 4).
-  const long long target = 4096;
-  for (int v : {2}) {
-    if (v * fb > 2) continue;
-    const long long blocks = ((n_pix + kThreads * v - 1) / (kThreads * v)) * n_fblocks;
-    if (blocks >= target) return v;
-  }
-  return 1;
+  // V = 2 keeps the kernel at 3 waves per SIMD (measured fastest); V = 1 when
+  // that would leave fewer than ~4 workgroups per CU (256 CUs on MI355X).
+  const long long blocks2 = ((n_pix + kThreads * 2 - 1) / (kThreads * 2)) * n_fblocks;
+  return (fb <= 4 && blocks2 >= 1024) ? 2 : 1;
 }
 
 template <int KIND, int V, int FB>
@@ -828,7 +716,7 @@ const char* rca_version(void) { return "rca_das 0.1.0 gfx950"; }
 // Tuning override for sweeps: workgroup voxels per lane (V), frames per block
 // (FB) and adjoint voxel splits; 0 restores the automatic choice.
 int rca_set_tuning(int v, int fb, int split) {
-  if ((v != 0 && v != 1 && v != 2) || (fb != 0 && fb != 1 && fb != 2 && fb != 4 && fb != 8) ||
+  if ((v != 0 && v != 1 && v != 2 && v != 4) || (fb != 0 && fb != 1 && fb != 2 && fb != 4 && fb != 8) ||
       split < 0)
     return fail(RCA_ERR_INVALID, "bad tuning V=%d FB=%d split=%d", v, fb, split);
   g_tune_v = v;
@@ -848,8 +736,10 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
     return fail(RCA_ERR_INVALID, "unknown iq dtype %d", iq_dtype);
   if (iq_dtype == RCA_IQ_F16X2 && !trace_scale)
     return fail(RCA_ERR_INVALID, "trace_scale is required for FP16 I/Q");
-  if (flags & ~(RCA_FLAG_COMPOUND | RCA_FLAG_ACCUMULATE))
+  if (flags & ~(RCA_FLAG_COMPOUND | RCA_FLAG_ACCUMULATE | RCA_FLAG_NOSUM))
     return fail(RCA_ERR_INVALID, "unknown flags 0x%x", flags);
+  if ((flags & RCA_FLAG_NOSUM) && (flags & RCA_FLAG_COMPOUND))
+    return fail(RCA_ERR_INVALID, "RCA_FLAG_NOSUM excludes RCA_FLAG_COMPOUND");
   if (n_frame == 0 || d->n_pix == 0) return RCA_OK;
   const size_t sbytes = iq_dtype == RCA_IQ_C64 ? 8 : 4;
   if ((size_t)d->n_sample * sbytes > (size_t)kMaxLds)
@@ -864,13 +754,14 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
   a.mask = mask;
   a.compound = (flags & RCA_FLAG_COMPOUND) != 0;
   a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
+  a.nosum = (flags & RCA_FLAG_NOSUM) != 0;
   a.vec_ok = (((uintptr_t)iq & 15) == 0) && (((size_t)d->n_sample * sbytes) % 16 == 0);
   int fb = g_tune_fb ? g_tune_fb : pick_fb(n_frame, d->n_sample, sbytes);
   if ((size_t)fb * d->n_sample * sbytes > (size_t)kMaxLds)
     return fail(RCA_ERR_UNSUPPORTED, "FB=%d does not fit in LDS", fb);
   const int nfb = (n_frame + fb - 1) / fb;
   int v = g_tune_v ? g_tune_v : pick_v(d->n_pix, nfb, fb);
-  if (v * fb > 4) v = 1;   // instantiated: V=2 for FB <= 2, V=1 for any FB
+  if ((v == 4 && fb > 1) || (v == 2 && fb > 4)) v = 1;   // instantiated shapes
   if (iq_dtype == RCA_IQ_C64) return launch_fwd_s<float2>(a, d->interp, v, fb, stream);
   return launch_fwd_s<__half2>(a, d->interp, v, fb, stream);
 }

@@ -202,17 +202,24 @@ class RcaPlan:
         return dtype, f
 
     def forward(self, iq, scale=None, *, compound=True, out=None, mask=None, want_mask=True,
-                accumulate=False):
-        """Beamform ``iq``; returns (out [F, 1|T, P] complex64, mask [1|T, P] or None)."""
+                accumulate=False, nosum=False):
+        """Beamform ``iq``; returns (out, mask).
+
+        out is complex64 [F, 1|T, P] (compounded or per tilt), or
+        [nChannel, F, T, P] with ``nosum`` (one output per receive line, the
+        reference's NoSum variant); mask is float32 [1|T, P] or None."""
         dtype, nf = self._check_iq(iq, scale)
+        if nosum and compound:
+            raise ValueError("nosum outputs are per tilt and per line: use compound=False")
         nt = 1 if compound else self.n_tx
+        shape = (self.n_channel, nf, nt, self.n_pix) if nosum else (nf, nt, self.n_pix)
         if out is None:
             if accumulate:
                 raise ValueError("accumulate needs an output tensor")
-            out = torch.empty((nf, nt, self.n_pix), dtype=torch.complex64, device=self.device)
+            out = torch.empty(shape, dtype=torch.complex64, device=self.device)
         elif (out.dtype != torch.complex64 or not out.is_contiguous() or out.device != self.device
-              or tuple(out.shape) != (nf, nt, self.n_pix)):
-            raise ValueError(f"out must be contiguous complex64 {(nf, nt, self.n_pix)} on {self.device}")
+              or tuple(out.shape) != shape):
+            raise ValueError(f"out must be contiguous complex64 {shape} on {self.device}")
         if want_mask and mask is None:
             if accumulate:
                 raise ValueError("accumulate with a mask needs the mask tensor")
@@ -220,7 +227,8 @@ class RcaPlan:
         if mask is not None and (mask.dtype != torch.float32 or not mask.is_contiguous()
                                  or tuple(mask.shape) != (nt, self.n_pix) or mask.device != self.device):
             raise ValueError(f"mask must be contiguous float32 {(nt, self.n_pix)}")
-        flags = (_lib.FLAG_COMPOUND if compound else 0) | (_lib.FLAG_ACCUMULATE if accumulate else 0)
+        flags = ((_lib.FLAG_COMPOUND if compound else 0) | (_lib.FLAG_ACCUMULATE if accumulate else 0)
+                 | (_lib.FLAG_NOSUM if nosum else 0))
         rc = self.lib.rca_forward(ctypes.byref(self.desc), ctypes.c_void_p(iq.data_ptr()), dtype,
                                   ctypes.c_void_p(scale.data_ptr() if scale is not None else 0),
                                   nf, ctypes.c_void_p(out.data_ptr()),

@@ -94,7 +94,7 @@ def test_launch_shapes_agree(dev):
     plan, G, T = plan_for(cfg, 0, dev, "lanczos4")
     outs = {}
     try:
-        for v in (1, 2):
+        for v in (1, 2, 4):
             for fb in (1, 2, 4):
                 _lib.set_tuning(v, fb, 0)
                 outs[(v, fb)] = plan.forward(iq)[0].cpu().numpy()
@@ -255,3 +255,62 @@ def test_dropin_numpy_torch_real(dev):
     ref4, rm4 = O.bf(data, G, T, P, Opn)
     assert got4.shape == ref4.shape == (20, 9, 7, 11, 2)
     check(got4, ref4, "drop-in no compounding")
+
+
+@pytest.mark.parametrize("interp", ["quadratic", "lanczos4"])
+def test_nosum_per_line_outputs(dev, interp):
+    cfg = C.scaled(SMALL, nx=5, ny=4, nz=16)
+    say(f"case NoSum per-line outputs interp={interp}")
+    iq = H.iq_frame(cfg, 0, n_frame=2)
+    plan, G, T = plan_for(cfg, 0, dev, interp)
+    out, mask = plan.forward(iq.to(dev), compound=False, nosum=True)
+    torch.cuda.synchronize()
+    G0, T0, P0, Op0 = H.case(cfg, 0)
+    kind, a = OKIND[interp]
+    ref, rmask = O.kernel_forward(iq.numpy().view(np.float32), H.grid_f32(G0), O.prepare(G0, T0),
+                                  H.oracle_params(cfg, cfg.halves[0]), kind, a, False, nosum=True)
+    assert tuple(out.shape) == ref.shape                     # [nCh, F, T, P]
+    r = H.rel_l2(out.cpu().numpy(), ref)
+    say(f"nosum rel-L2 {r:.3e}")
+    assert r <= REL
+    assert H.rel_l2(mask.cpu().numpy(), rmask) <= REL
+    # the per-line outputs sum to the per-tilt output
+    full, _ = plan.forward(iq.to(dev), compound=False)
+    assert H.rel_l2(out.sum(dim=0).cpu().numpy(), full.cpu().numpy()) <= 1e-5
+
+
+def test_dropin_variants(dev):
+    from simu_rca_ab_amd.beamformer import (bfVirtualSourceGeneric_RCA_Nosum,
+                                            bfVirtualSourceGeneric_RCA_Nosum_ab,
+                                            bfVirtualSourceGeneric_RCA_ab_1,
+                                            bfVirtualSourceGeneric_RCA_ab_2)
+    cfg = C.scaled(C.get("cfg0"), nx=4, ny=3, nz=10)
+    say("case drop-in variants NoSum / ab_1 / ab_2 / Nosum_ab")
+    iq = H.iq_frame(cfg, 0, n_frame=2)
+    G, T, P, Op = H.case(cfg, 0)
+    data = synth.to_matlab_layout(iq.numpy())
+    grid = H.grid_f32(G)
+    prep = O.prepare(G, T)
+    prm = H.oracle_params(cfg, cfg.halves[0])
+    d, m = bfVirtualSourceGeneric_RCA_Nosum(data, G, T, P, Op)
+    assert d.shape == (10, 4, 3, 11, 2, 64) and m.shape == (10, 4, 3, 11)
+    ref, _ = O.kernel_forward(iq.numpy().view(np.float32), grid, prep, prm, O.INTERP_LANCZOS, 3,
+                              nosum=True)
+    # [P, T, F, C] (column-major voxel flatten) -> [C, F, T, P]
+    check(np.moveaxis(d.reshape(-1, 11, 2, 64, order="F"), [3, 2, 1, 0], [0, 1, 2, 3]), ref,
+          "Nosum drop-in")
+    law1 = (np.random.default_rng(0).standard_normal(64) * 15e-9)
+    law2 = (np.random.default_rng(1).standard_normal((64, 11)) * 15e-9)   # AbLawRx(:)[tx*64+ch]
+    for fn, law, mode, nos in ((bfVirtualSourceGeneric_RCA_ab_1, law1, 1, False),
+                               (bfVirtualSourceGeneric_RCA_ab_2, law2, 2, False),
+                               (bfVirtualSourceGeneric_RCA_Nosum_ab, law1, 1, True)):
+        d, _ = fn(data, G, T, dict(P, AbLawRx=law), Op)
+        ref, _ = O.kernel_forward(iq.numpy().view(np.float32), grid, prep, prm, O.INTERP_LANCZOS, 3,
+                                  nosum=nos, ablaw=np.asarray(law, np.float32).ravel(order="F"),
+                                  ablaw_mode=mode)
+        if nos:
+            got = np.moveaxis(d.reshape(-1, 11, 2, 64, order="F"), [3, 2, 1, 0], [0, 1, 2, 3])
+        else:
+            got = d[..., 0, :].reshape(-1, 2, order="F").T
+            ref = ref.sum(axis=1)
+        check(got, ref, f"{fn.__name__}")
