@@ -117,6 +117,64 @@ def run_fp16(cfg, dev, n_frame=16, reps=3):
     return res
 
 
+def run_serving(dev, capacity, n_req=32, load=0.7, seed=3000):
+    """Serving stream: Poisson arrivals of requests drawn like the serving
+    workload (grid N_xy ~ U{64..160}, N_z = 2 N_xy, tilts in {8,16,32,64},
+    FP32 or FP16 I/Q) with 1-4 frames per request, offered at ``load`` times
+    the plain kernel's measured throughput."""
+    import numpy as np
+    from simu_rca_ab_amd.serving import GpuBackend, RcaServer, RequestSpec, synth_request
+    rng = np.random.Generator(np.random.PCG64(seed))
+    be = GpuBackend(dev)
+    specs = []
+    for _ in range(n_req):
+        nxy = int(rng.integers(64, 161))
+        specs.append(RequestSpec(nxy, nxy, 2 * nxy, int(rng.choice([8, 16, 32, 64])),
+                                 int(rng.integers(1, 5)), "fp16" if rng.random() < 0.5 else "fp32"))
+    log(f"[bench] serving: synthesising {n_req} requests")
+    reqs = [synth_request(i, sp, be, dev, n_scat=100) for i, sp in enumerate(specs)]
+    mean_upd = float(np.mean([sp.updates(be.n_elem) for sp in specs]))
+    rate = load * capacity / mean_upd
+    gaps = rng.exponential(1.0 / rate, n_req)
+    srv = RcaServer(be, budget_bytes=48 << 30).start()
+    log(f"[bench] serving: {n_req} requests at {rate:.2f} req/s (offered load {load:.0%})")
+    t0 = time.perf_counter()
+    t = t0
+    for r, g in zip(reqs, gaps):
+        t += g
+        while time.perf_counter() < t:
+            time.sleep(min(1e-3, max(0.0, t - time.perf_counter())))
+        srv.submit(r)
+    for r in reqs:
+        r.wait(600)
+    srv.close(60)
+    st = srv.stats.summary()
+    st.update({"arrival_rate_per_s": rate, "offered_load": load, "n_requests": n_req,
+               "frames_per_request": "1-4", "mean_updates_per_request": mean_upd,
+               "fp16_fraction": sum(sp.dtype == "fp16" for sp in specs) / n_req,
+               "latency": "arrival -> beamformed volume in host memory"})
+    return st
+
+
+def run_dbf(dev, steps=3):
+    """Differentiable beamforming (96+96 elements, 16 tilts, 96x96x192 voxels,
+    8 frames, Lanczos a=3) feeding a 4-layer complex 3-D CNN (32 channels)."""
+    import torch
+    from simu_rca_ab_amd import configs as C, dbf, synth
+    cfg = C.get("cfg4")
+    plan = build_plans(cfg, dev)[0]
+    one = synth.frame(cfg, device=dev)[0]
+    ph = torch.exp(1j * torch.linspace(0, 2, cfg.n_frame, device=dev)).to(torch.complex64)
+    iq = (one * ph[:, None, None, None]).contiguous()
+    r = dbf.time_cfg4(plan, iq, (cfg.nx, cfg.ny, cfg.nz), steps=steps)
+    upd = r["updates_per_pass"]
+    r["das_forward_updates_per_s"] = upd / (r["das_forward_ms"] * 1e-3)
+    r["das_adjoint_updates_per_s"] = upd / (r["das_adjoint_ms"] * 1e-3)
+    r["workload"] = ("cfg4: 96+96 elements, 16 tilts -10..10 deg, 1024 samples, 96x96x192 voxels, "
+                     "8 frames, Lanczos a=3 -> 4 complex Conv3d (32 ch, 3^3, CReLU) -> MSE")
+    return r
+
+
 def cpu_baseline(cfg, target_s=6.0):
     """FP32 CPU restatement of the reference kernel on a voxel sample of the
     workload, OpenMP over every core this process may use."""
@@ -164,6 +222,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-I/Q leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--no-dbf", action="store_true", help="skip the differentiable-beamforming leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device copy (HBM-counter calibration)")
     args = ap.parse_args()
@@ -204,6 +264,14 @@ def main():
         extra["fp16_iq"] = {"value": r["fp16"], "fp32_same_ensemble": r["fp32"],
                             "unit": "voxel*tilt*element updates/s", "frames": 16,
                             "vs_plain_fp32_single_frame": r["fp16"] / value}
+    if not args.no_dbf:
+        log("[bench] differentiable beamforming leg")
+        extra["dbf"] = run_dbf(dev)
+        torch.cuda.empty_cache()
+    if not args.no_serving:
+        log("[bench] serving leg")
+        extra["serving"] = run_serving(dev, value)
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline leg")

@@ -127,6 +127,41 @@ int rca_adjoint(const rca_das_desc* desc, const void* grad_out, int grad_tx, int
                 void* grad_iq, int flags, hipStream_t stream);
 
 /*
+ * Batched forward for serving (no reference counterpart; the reference makes
+ * one synchronous call per chunk, bfVirtualSourceGeneric_RCA.m:664-711).
+ * A batch is a list of entries; each entry beamforms frames
+ * [frame_begin, frame_end) of one request: one or two RCA halves on the same
+ * voxel grid, compounded over tilts and summed over halves into `out`
+ * (complex64 [n_frame][1][nPix]) and `mask` ([1][nPix], written by the batch
+ * that contains frame 0).  rca_batch_encode fills a host workspace (header,
+ * per-half arguments, work items); the caller copies it to a device buffer of
+ * the same size; rca_batch_launch runs a persistent grid of grid_blocks
+ * workgroups over the device work list.  The launch arguments never change
+ * between batches, so one captured HIP graph replays every batch.  Per-voxel
+ * arithmetic is the same as rca_forward's: results are bitwise identical
+ * whatever else shares the batch.  FP16 entries need frame_begin even and
+ * frame_end even or equal to n_frame.
+ */
+typedef struct {
+  const rca_das_desc* half[2];
+  int n_half;                  /* 1 or 2 */
+  const void* iq[2];           /* per half, [n_frame][nTx][nChannel][nSample] */
+  const float* trace_scale[2]; /* per half, FP16 only */
+  int iq_dtype;                /* rca_iq_dtype */
+  int n_frame;
+  int frame_begin;
+  int frame_end;
+  void* out;
+  float* mask;                 /* may be NULL */
+} rca_batch_entry;
+
+size_t rca_batch_workspace_bytes(int max_halves, int max_items);
+int rca_batch_encode(const rca_batch_entry* entries, int n_entries, int max_halves, int max_items,
+                     int n_sample_max, void* host_ws, size_t ws_bytes, int* n_items);
+int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_blocks,
+                     int n_sample_max, hipStream_t stream);
+
+/*
  * Launch-shape override for tuning sweeps (no reference counterpart):
  * voxels per lane v in {1,2,4}, frames per workgroup fb in {1,2,4,8}, adjoint
  * voxel splits; 0 restores the automatic choice.  Process-wide.

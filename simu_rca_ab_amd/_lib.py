@@ -20,7 +20,8 @@ ABLAW = {None: 0, "channel": 1, "tx_channel": 2}
 
 # Every symbol include/rca_das.h declares.
 EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last_error",
-           "rca_version", "rca_set_tuning")
+           "rca_version", "rca_set_tuning", "rca_batch_workspace_bytes", "rca_batch_encode",
+           "rca_batch_launch")
 
 
 class RcaDasDesc(ctypes.Structure):

@@ -340,15 +340,15 @@ __device__ __forceinline__ void ab_terms(const KArgs& a, int t, int c, double& u
 // registers (2 waves/SIMD).
 // ----------------------------------------------------------------------------
 template <int KIND, int V, int FB, typename S>
-__global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
+__device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned by,
+                                          unsigned char* smem_raw) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
   S* tr = reinterpret_cast<S*>(smem_raw);
   const S* iq = reinterpret_cast<const S*>(a.iq);
 
   const int tid = threadIdx.x;
-  const long long pblk = (long long)blockIdx.x * (kThreads * V);
-  const int f0 = blockIdx.y * FB;
+  const long long pblk = (long long)bx * (kThreads * V);
+  const int f0 = by * FB;
   const int nf = min(FB, a.n_frame - f0);
   const int ns = a.n_sample;
   const long long fstride = (long long)a.n_tx * a.n_ch * ns;
@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
             *o = r;
           }
         }
-        if (a.mask && blockIdx.y == 0) {
+        if (a.mask && by == 0) {
           float* mk = a.mask + (long long)t * a.n_pix + p;
           *mk = a.accumulate ? *mk + mT[v] : mT[v];
         }
@@ -505,10 +505,53 @@ __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
           *o = r;
         }
       }
-      if (a.mask && blockIdx.y == 0) {
+      if (a.mask && by == 0) {
         float* mk = a.mask + p;
         *mk = a.accumulate ? *mk + macc[v] : macc[v];
       }
+    }
+  }
+}
+
+template <int KIND, int V, int FB, typename S>
+__global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  fwd_block<KIND, V, FB, S>(a, blockIdx.x, blockIdx.y, smem_raw);
+}
+
+// ----------------------------------------------------------------------------
+// Batched forward for the serving loop.  A persistent grid walks a work list
+// kept in device memory (so one captured launch serves batches of any shape):
+//   header  { n_items }
+//   args    KArgs per request half (pointers, sizes, flags; iq_dtype below)
+//   items   { first arg, number of halves (1|2), voxel block, frame block }
+// Workgroup g takes items g, g + gridDim.x, ...  Both halves of an item run in
+// the same workgroup, the second accumulating onto the first, so items never
+// race on an output.  FP32 items use FB = 1 and FP16 items FB = 2.
+// ----------------------------------------------------------------------------
+struct BatchArgs {
+  KArgs k;
+  int iq_dtype;
+  int pad_[3];
+};
+
+constexpr int kBatchV = 2;
+
+template <int KIND>
+__global__ void __launch_bounds__(kThreads) das_fwd_batch(const int* __restrict__ header,
+                                                          const BatchArgs* __restrict__ args,
+                                                          const int4* __restrict__ items) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = header[0];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int4 it = items[i];
+    for (int h = 0; h < it.y; ++h) {
+      const KArgs k = args[it.x + h].k;
+      if (args[it.x + h].iq_dtype == RCA_IQ_C64)
+        fwd_block<KIND, kBatchV, 1, float2>(k, it.z, it.w, smem_raw);
+      else
+        fwd_block<KIND, kBatchV, 2, __half2>(k, it.z, it.w, smem_raw);
+      __syncthreads();
     }
   }
 }
@@ -845,6 +888,106 @@ int bfVirtualSourceGeneric_RCA(float* dataBf, float* mask, const float* data,
   d.speed_of_sound = speedOfSound; d.start_time = startTime; d.fnumber = fnumber;
   if (!dataBf || !mask) return fail(RCA_ERR_INVALID, "dataBf/mask pointers are NULL");
   return rca_forward(&d, data, RCA_IQ_C64, nullptr, nFrame, dataBf, mask, 0, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches for the serving loop (persistent, graph-capturable).
+// ---------------------------------------------------------------------------
+static size_t batch_args_off() { return 256; }
+static size_t batch_items_off(int max_halves) {
+  return (batch_args_off() + (size_t)max_halves * sizeof(BatchArgs) + 255) / 256 * 256;
+}
+
+size_t rca_batch_workspace_bytes(int max_halves, int max_items) {
+  if (max_halves < 1 || max_items < 1) return 0;
+  return batch_items_off(max_halves) + (size_t)max_items * sizeof(int4);
+}
+
+int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, int max_items,
+                     int n_sample_max, void* host_ws, size_t ws_bytes, int* n_items_out) {
+  g_err.clear();
+  if (!host_ws || !n_items_out || n_entries < 0)
+    return fail(RCA_ERR_INVALID, "null workspace / counter or negative entry count");
+  if (ws_bytes < rca_batch_workspace_bytes(max_halves, max_items))
+    return fail(RCA_ERR_INVALID, "workspace too small for %d halves / %d items", max_halves, max_items);
+  unsigned char* ws = reinterpret_cast<unsigned char*>(host_ws);
+  BatchArgs* args = reinterpret_cast<BatchArgs*>(ws + batch_args_off());
+  int4* items = reinterpret_cast<int4*>(ws + batch_items_off(max_halves));
+  int na = 0, ni = 0;
+  for (int q = 0; q < n_entries; ++q) {
+    const rca_batch_entry& E = e[q];
+    if (E.n_half < 1 || E.n_half > 2) return fail(RCA_ERR_INVALID, "entry %d: n_half must be 1 or 2", q);
+    if (E.iq_dtype != RCA_IQ_C64 && E.iq_dtype != RCA_IQ_F16X2)
+      return fail(RCA_ERR_INVALID, "entry %d: unknown iq dtype", q);
+    const int fb = E.iq_dtype == RCA_IQ_C64 ? 1 : 2;
+    if (!E.out || E.n_frame < 1 || E.frame_begin < 0 || E.frame_end > E.n_frame ||
+        E.frame_begin >= E.frame_end || E.frame_begin % fb ||
+        (E.frame_end % fb && E.frame_end != E.n_frame))
+      return fail(RCA_ERR_INVALID, "entry %d: bad output or frame range [%d, %d) of %d", q,
+                  E.frame_begin, E.frame_end, E.n_frame);
+    if (na + E.n_half > max_halves) return fail(RCA_ERR_UNSUPPORTED, "batch exceeds %d halves", max_halves);
+    const int first = na;
+    int n_pix = -1;
+    for (int h = 0; h < E.n_half; ++h) {
+      const rca_das_desc* d = E.half[h];
+      int rc = check_desc(d);
+      if (rc) return rc;
+      if (n_pix >= 0 && d->n_pix != n_pix) return fail(RCA_ERR_INVALID, "entry %d: halves differ in nPix", q);
+      n_pix = d->n_pix;
+      if (d->n_sample > n_sample_max)
+        return fail(RCA_ERR_UNSUPPORTED, "entry %d: nSample %d > batch maximum %d", q, d->n_sample, n_sample_max);
+      if (!E.iq[h] || (E.iq_dtype == RCA_IQ_F16X2 && !E.trace_scale[h]))
+        return fail(RCA_ERR_INVALID, "entry %d: missing I/Q or scale of half %d", q, h);
+      KArgs a = make_args(d, E.n_frame);
+      a.iq = E.iq[h];
+      a.scale = E.trace_scale[h];
+      a.out = reinterpret_cast<float2*>(E.out);
+      a.mask = E.mask;
+      a.compound = 1;
+      a.accumulate = h > 0;
+      a.nosum = 0;
+      const size_t sb = E.iq_dtype == RCA_IQ_C64 ? 8 : 4;
+      a.vec_ok = (((uintptr_t)E.iq[h] & 15) == 0) && (((size_t)d->n_sample * sb) % 16 == 0);
+      args[na].k = a;
+      args[na].iq_dtype = E.iq_dtype;
+      ++na;
+    }
+    const int nvb = (n_pix + kThreads * kBatchV - 1) / (kThreads * kBatchV);
+    for (int fbk = E.frame_begin / fb; fbk * fb < E.frame_end; ++fbk) {
+      for (int vb = 0; vb < nvb; ++vb) {
+        if (ni >= max_items) return fail(RCA_ERR_UNSUPPORTED, "batch exceeds %d items", max_items);
+        items[ni++] = make_int4(first, E.n_half, vb, fbk);
+      }
+    }
+  }
+  reinterpret_cast<int*>(ws)[0] = ni;
+  *n_items_out = ni;
+  return RCA_OK;
+}
+
+int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_blocks,
+                     int n_sample_max, hipStream_t stream) {
+  g_err.clear();
+  if (!dev_ws || max_halves < 1 || grid_blocks < 1 || n_sample_max < 1)
+    return fail(RCA_ERR_INVALID, "bad batch launch arguments");
+  const size_t lds = (size_t)n_sample_max * 8;
+  if (lds > (size_t)kMaxLds) return fail(RCA_ERR_UNSUPPORTED, "nSample %d exceeds LDS", n_sample_max);
+  const unsigned char* ws = reinterpret_cast<const unsigned char*>(dev_ws);
+  const int* hdr = reinterpret_cast<const int*>(ws);
+  const BatchArgs* args = reinterpret_cast<const BatchArgs*>(ws + batch_args_off());
+  const int4* items = reinterpret_cast<const int4*>(ws + batch_items_off(max_halves));
+  switch (interp) {
+#define RCA_BATCH_CASE(K) \
+    case K: hipLaunchKernelGGL(das_fwd_batch<K>, dim3(grid_blocks), dim3(kThreads), lds, stream, hdr, args, items); break;
+    RCA_BATCH_CASE(RCA_INTERP_QUADRATIC)
+    RCA_BATCH_CASE(RCA_INTERP_LANCZOS_REF)
+    RCA_BATCH_CASE(RCA_INTERP_LANCZOS3)
+    RCA_BATCH_CASE(RCA_INTERP_LANCZOS4)
+#undef RCA_BATCH_CASE
+    default: return fail(RCA_ERR_INVALID, "unknown interpolation %d", interp);
+  }
+  const hipError_t err = hipGetLastError();
+  return err == hipSuccess ? RCA_OK : hip_fail(err, "das_fwd_batch launch");
 }
 
 }  // extern "C"

@@ -1,0 +1,137 @@
+"""Scheduler / admission logic of the serving loop on the CPU, through a fake
+backend with the GPU backend's interface (no GPU work)."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from simu_rca_ab_amd.serving import Request, RequestSpec, RcaServer
+
+
+class _Ev:
+    def __init__(self, t_ready):
+        self.t = t_ready
+
+    def query(self):
+        return time.perf_counter() >= self.t
+
+    def synchronize(self):
+        while not self.query():
+            time.sleep(1e-4)
+
+
+class FakeBackend:
+    """Executes batches 'instantly' after a simulated delay proportional to work;
+    the result of frame f of request r is r*1000 + f in every voxel."""
+    n_elem = 4
+
+    def __init__(self, upd_per_s=1e9, fail_on_batch=None):
+        self.rate = upd_per_s
+        self.busy_until = 0.0
+        self.batches = []
+        self.fail_on_batch = fail_on_batch
+        self.live = {}
+
+    def request_bytes(self, spec):
+        return spec.n_vox * spec.n_frame * 8
+
+    def upload(self, req):
+        req.dev = {"out": np.zeros((req.spec.n_frame, req.spec.n_vox), np.complex64)}
+        self.live[req.rid] = True
+        return _Ev(time.perf_counter() + 1e-3)
+
+    def run_batch(self, k, work):
+        if self.fail_on_batch is not None and len(self.batches) == self.fail_on_batch:
+            raise RuntimeError("injected launch failure")
+        self.batches.append([(r.rid, f0, f1) for r, f0, f1 in work])
+        upd = 0
+        for r, f0, f1 in work:
+            assert self.live.get(r.rid), "work issued for a released request"
+            for f in range(f0, f1):
+                r.dev["out"][f] = r.rid * 1000 + f
+            upd += r.spec.updates(self.n_elem) // r.spec.n_frame * (f1 - f0)
+        start = max(self.busy_until, time.perf_counter())
+        self.busy_until = start + upd / self.rate
+        return _Ev(self.busy_until), 0
+
+    def download(self, req, after):
+        return req.dev["out"].copy(), _Ev(after.t + 1e-4)
+
+    def release(self, req):
+        self.live[req.rid] = False
+        req.dev = {}
+
+    def max_frames_per_chunk(self, spec, quantum):
+        per = spec.updates(self.n_elem) // spec.n_frame
+        return max(1, quantum // per)
+
+
+def _req(rid, nvox=1000, frames=4, tilts=8):
+    return Request(rid, RequestSpec(nvox, 1, 1, tilts, frames), halves=[None, None])
+
+
+def test_all_requests_complete_with_their_own_results():
+    be = FakeBackend()
+    srv = RcaServer(be, budget_bytes=10**9, quantum=10**6).start()
+    reqs = [srv.submit(_req(i, nvox=500 + 100 * i, frames=1 + i % 5)) for i in range(12)]
+    for r in reqs:
+        out = r.wait(10)
+        want = r.rid * 1000 + np.arange(r.spec.n_frame)[:, None]
+        assert np.array_equal(out.real, np.broadcast_to(want, out.shape))
+    srv.drain(5)
+    srv.close(5)
+    s = srv.stats.summary()
+    assert s["requests"] == 12 and s["rejected"] == 0 and s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_admission_respects_budget_and_order():
+    be = FakeBackend(upd_per_s=2e7)
+    spec_bytes = 1000 * 4 * 8
+    srv = RcaServer(be, budget_bytes=2 * spec_bytes, quantum=10**9).start()
+    reqs = [srv.submit(_req(i)) for i in range(5)]
+    for r in reqs:
+        r.wait(20)
+    srv.close(5)
+    assert srv.stats.max_bytes <= 2 * spec_bytes
+    assert srv.stats.max_in_flight <= 2
+    adm = [r.admitted for r in reqs]
+    assert adm == sorted(adm)                       # first come, first served
+    assert reqs[2].admitted >= min(reqs[0].done, reqs[1].done) - 1e-3
+
+
+def test_oversized_request_is_rejected():
+    be = FakeBackend()
+    srv = RcaServer(be, budget_bytes=1000, quantum=10**6).start()
+    big = srv.submit(_req(0, nvox=10**5))
+    small = srv.submit(_req(1, nvox=10, frames=1))
+    with pytest.raises(RuntimeError, match="budget"):
+        big.wait(5)
+    assert small.wait(5) is not None
+    srv.close(5)
+    assert srv.stats.rejected == 1
+
+
+def test_quantum_splits_large_requests_and_interleaves():
+    be = FakeBackend(upd_per_s=1e6)          # 32 ms of simulated GPU time per frame
+    per_frame = 1000 * 8 * 4
+    srv = RcaServer(be, budget_bytes=10**9, quantum=2 * per_frame).start()
+    a = srv.submit(_req(0, frames=8))
+    time.sleep(0.02)
+    b = srv.submit(_req(1, frames=2))
+    a.wait(10), b.wait(10)
+    srv.close(5)
+    for batch in be.batches:
+        frames = sum(f1 - f0 for _, f0, f1 in batch)
+        assert frames <= 2
+    # the late request finished before the long one: it joined a batch mid-way
+    assert b.done < a.done
+
+
+def test_backend_failure_reaches_waiters():
+    be = FakeBackend(fail_on_batch=0)
+    srv = RcaServer(be, budget_bytes=10**9, quantum=10**6).start()
+    r = srv.submit(_req(0))
+    with pytest.raises(RuntimeError, match="server failed"):
+        r.wait(5)
+    srv.close(5)
