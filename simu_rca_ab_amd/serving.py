@@ -294,14 +294,14 @@ class GpuBackend:
         req.dev = {}
 
     def max_frames_per_chunk(self, spec: RequestSpec, quantum: int) -> int:
+        """Frames per work chunk: about ``quantum`` updates, within the work-list
+        capacity; even for FP16 requests (the batched kernel runs FP16 frames
+        in pairs, so chunks must start on an even frame)."""
         per_frame = spec.updates(self.n_elem) // spec.n_frame
-        f = max(1, quantum // max(per_frame, 1))
-        if spec.dtype == "fp16" and f > 1:
-            f -= f % 2
         items_per_frame = math.ceil(spec.n_vox / 512)
-        f = max(1, min(f, self.max_items // max(items_per_frame, 1)))
-        if spec.dtype == "fp16" and f > 1:
-            f -= f % 2
+        f = max(1, min(quantum // max(per_frame, 1), self.max_items // max(items_per_frame, 1)))
+        if spec.dtype == "fp16":
+            f = max(2, f - f % 2)
         return f
 
 
