@@ -29,6 +29,8 @@ from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+# MIOpen: heuristic solver choice instead of an exhaustive search on first use
+os.environ.setdefault("MIOPEN_FIND_MODE", "FAST")
 
 HBM_PEAK_GBS = 8000.0
 
@@ -166,7 +168,7 @@ def run_dbf(dev, steps=3):
     one = synth.frame(cfg, device=dev)[0]
     ph = torch.exp(1j * torch.linspace(0, 2, cfg.n_frame, device=dev)).to(torch.complex64)
     iq = (one * ph[:, None, None, None]).contiguous()
-    r = dbf.time_cfg4(plan, iq, (cfg.nx, cfg.ny, cfg.nz), steps=steps)
+    r = dbf.time_cfg4(plan, iq, (cfg.nx, cfg.ny, cfg.nz), steps=steps, log=log)
     upd = r["updates_per_pass"]
     r["das_forward_updates_per_s"] = upd / (r["das_forward_ms"] * 1e-3)
     r["das_adjoint_updates_per_s"] = upd / (r["das_adjoint_ms"] * 1e-3)
@@ -223,7 +225,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-I/Q leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream leg")
-    ap.add_argument("--no-dbf", action="store_true", help="skip the differentiable-beamforming leg")
+    ap.add_argument("--dbf", action="store_true",
+                    help="add the differentiable-beamforming leg (its first run compiles MIOpen "
+                         "Conv3d kernels, which can take minutes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device copy (HBM-counter calibration)")
     args = ap.parse_args()
@@ -264,7 +268,7 @@ def main():
         extra["fp16_iq"] = {"value": r["fp16"], "fp32_same_ensemble": r["fp32"],
                             "unit": "voxel*tilt*element updates/s", "frames": 16,
                             "vs_plain_fp32_single_frame": r["fp16"] / value}
-    if not args.no_dbf:
+    if args.dbf:
         log("[bench] differentiable beamforming leg")
         extra["dbf"] = run_dbf(dev)
         torch.cuda.empty_cache()

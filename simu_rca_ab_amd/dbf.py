@@ -65,7 +65,7 @@ def train_step(iq, plan, net, target, opt, grid_shape):
     return loss.detach()
 
 
-def time_cfg4(plan, iq, grid_shape, steps=3, ch=32, seed=4004):
+def time_cfg4(plan, iq, grid_shape, steps=3, ch=32, seed=4004, log=None):
     """Times forward+backward training steps and the adjoint (backward of the
     beamformer) alone.  Returns a dict of milliseconds."""
     dev = iq.device
@@ -78,11 +78,14 @@ def time_cfg4(plan, iq, grid_shape, steps=3, ch=32, seed=4004):
               torch.randn(tshape, device=dev, generator=g) * 0.1)
     opt = torch.optim.SGD(net.parameters(), lr=1e-3)
     x = iq.clone().requires_grad_(True)
-    losses = [train_step(x, plan, net, target, opt, grid_shape).item()]   # warm-up
+    say = log or (lambda *a: None)
+    say("[dbf] warm-up step (first use compiles the convolution kernels)")
+    losses = [train_step(x, plan, net, target, opt, grid_shape).item()]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
         losses.append(train_step(x, plan, net, target, opt, grid_shape).item())
+        say(f"[dbf] step {i + 1}/{steps} loss {losses[-1]:.5g}")
     torch.cuda.synchronize()
     step_ms = (time.perf_counter() - t0) * 1e3 / steps
     gout = torch.randn((iq.shape[0], 1, plan.n_pix), dtype=torch.complex64, device=dev, generator=g)
