@@ -33,6 +33,19 @@ sys.path.insert(0, str(ROOT))
 os.environ.setdefault("MIOPEN_FIND_MODE", "FAST")
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = ROOT / "profiles" / "das_fwd_cfg1_counters.md"
+
+
+def measured_traffic():
+    """HBM bytes per das_fwd launch at config 1 from the committed
+    FETCH_SIZE/WRITE_SIZE summary (rocprofv3 --pmc cannot run inside the
+    timed process), or None."""
+    import re
+    try:
+        m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+-]+)", TRAFFIC_PROFILE.read_text())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def log(*a):
@@ -292,7 +305,9 @@ def main():
                    "voxels": cfg.n_vox, "tilts": cfg.n_tilt, "elements": cfg.n_elem,
                    "samples": cfg.n_sample, "interp": cfg.interpolation, "frames": 1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "traffic_source": "profiles/das_fwd_cfg1_counters.md (FETCH_SIZE x 2.00 + "
+                                       "WRITE_SIZE x 1.00, copy-kernel calibrated)",
                      "kernel": "das_fwd<lanczos4>", "launch_ms": mean_launch,
                      "bytes_per_update": cfg.taps() * 8, "updates_per_launch": upd_launch},
         "cpu_baseline": cpu,
