@@ -62,7 +62,7 @@ def build_plans(cfg, dev):
         for k in ("gridBfX", "gridBfY", "gridBfZ"):
             G[k] = G[k].ravel(order="F")
         plans.append(RcaPlan.from_reference(G, C.transmit(h), C.param(cfg), C.options(cfg),
-                                            n_sample=cfg.n_sample, device=dev))
+                                            n_sample=cfg.n_sample, device=dev, depth_run=cfg.nz))
     return plans
 
 

@@ -84,6 +84,13 @@ typedef struct {
   float speed_of_sound;
   float start_time;
   float fnumber;
+  /* Voxel layout hint, no counterpart in the reference: when the voxel list is
+   * a grid stored depth-fastest (voxel p = iz + depth_run * column, MATLAB's
+   * [Nz, Nx, Ny] order) pass Nz here (>= 64) and each workgroup takes a tile
+   * of 64 depths x a few columns, which narrows the sample window it stages
+   * per (tilt, line).  0 = no hint (workgroups take consecutive voxels).  The
+   * results do not depend on it. */
+  int depth_run;
 } rca_das_desc;
 
 /*

@@ -32,7 +32,8 @@ class RcaDasDesc(ctypes.Structure):
         "ab_law_mode", "n_pix", "n_channel", "n_tx", "n_sample", "receive_x", "mask_transmit",
         "interp")] + [(n, ctypes.c_float) for n in (
         "aperture_x", "aperture_y", "aperture_z", "sampling_frequency",
-        "demodulation_frequency", "speed_of_sound", "start_time", "fnumber")]
+        "demodulation_frequency", "speed_of_sound", "start_time", "fnumber")] + [
+        ("depth_run", ctypes.c_int)]
 
 
 class RcaError(RuntimeError):

@@ -319,7 +319,8 @@ def _run(args, device, nosum, ablaw):
         law = (law.detach().cpu().numpy() if isinstance(law, torch.Tensor) else np.asarray(law))
         Par["AbLawRx"] = law.astype(np.float32).ravel(order="F")   # single(Param.AbLawRx)
         Par["AbLawMode"] = ablaw
-    plan = RcaPlan.from_reference(Geo, dict(Transmit), Par, Options, n_sample=ns_iq, device=device)
+    plan = RcaPlan.from_reference(Geo, dict(Transmit), Par, Options, n_sample=ns_iq, device=device,
+                                  depth_run=grid_shape[0])
 
     compound = bool(Options["flagCompounding"]) and not nosum
     out, mask = plan.forward(iq, compound=compound, nosum=nosum)

@@ -79,6 +79,7 @@ struct KArgs {
   int n_pix, n_ch, n_tx, n_sample, n_frame;
   int receive_x, mask_tx, ab_mode, compound, accumulate, vec_ok, nosum;
   int gout_tx, n_split, atomic_flush;
+  int depth_run, wcap;   // voxel tiling hint; LDS samples per frame row
 };
 
 // ----------------------------------------------------------------------------
@@ -339,38 +340,198 @@ __device__ __forceinline__ void ab_terms(const KArgs& a, int t, int c, double& u
 // variants that reuse the receive terms across tilts at the cost of more
 // registers (2 waves/SIMD).
 // ----------------------------------------------------------------------------
+// Voxel of lane (wave w, lane l), slot v in workgroup bx.  With a depth-run
+// hint (the grid's Nz, voxel p = iz + Nz*column) a workgroup owns a tile of
+// 64 depths x 4V columns, so its sample window per (tilt, line) is narrow;
+// otherwise it owns 256V consecutive voxels.
+__device__ __forceinline__ long long voxel_of(const KArgs& a, unsigned bx, int V, int v, int tid,
+                                              bool& valid) {
+  if (a.depth_run >= 64) {
+    const long long ndc = (a.depth_run + 63) / 64;
+    const long long dc = bx % ndc, cg = bx / ndc;
+    const long long col = cg * (4 * V) + (tid >> 6) * V + v;
+    const long long dep = dc * 64 + (tid & 63);
+    const long long p = dep + (long long)a.depth_run * col;
+    valid = dep < a.depth_run && p < a.n_pix;
+    return p;
+  }
+  const long long p = (long long)bx * (kThreads * V) + (long long)v * kThreads + tid;
+  valid = p < a.n_pix;
+  return p;
+}
+
+__host__ __device__ inline long long fwd_blocks(long long n_pix, int depth_run, int V) {
+  if (depth_run >= 64) {
+    const long long ncol = (n_pix + depth_run - 1) / depth_run;
+    return ((ncol + 4 * V - 1) / (4 * V)) * ((depth_run + 63) / 64);
+  }
+  return (n_pix + kThreads * V - 1) / (kThreads * V);
+}
+
+// Values every lane of the workgroup holds alike, moved to scalar registers.
+__device__ __forceinline__ float uni(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double uni(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// |t| over t in [lo, hi]: smallest and largest.
+__device__ __forceinline__ void abs_range(double lo, double hi, double& amin, double& amax) {
+  amin = (lo <= 0.0 && hi >= 0.0) ? 0.0 : fmin(fabs(lo), fabs(hi));
+  amax = fmax(fabs(lo), fabs(hi));
+}
+
+// Copy samples [s0, s0 + len) of the nf traces of one (tilt, line) into LDS
+// rows of wcap samples.
+template <typename S>
+__device__ __forceinline__ void stage_window(S* __restrict__ dst, const S* __restrict__ src,
+                                             long long fstride, int nf, int s0, int len, int wcap,
+                                             bool vec) {
+  if (vec) {   // s0 and the row length are multiples of 16 bytes
+    const int n16 = (int)(((long long)len * sizeof(S) + 15) / 16);
+    for (int fi = 0; fi < nf; ++fi) {
+      const uint4* s = reinterpret_cast<const uint4*>(src + fi * fstride + s0);
+      uint4* d = reinterpret_cast<uint4*>(dst + (long long)fi * wcap);
+      for (int i = threadIdx.x; i < n16; i += kThreads) d[i] = s[i];
+    }
+  } else {
+    for (int fi = 0; fi < nf; ++fi)
+      for (int i = threadIdx.x; i < len; i += kThreads) dst[fi * wcap + i] = src[fi * fstride + s0 + i];
+  }
+}
+
+// Adds one (tilt, line) to the partial sums of a lane's V voxels.  Taps come
+// from a staged window (FROM_LDS: samples [s0, s_end) of each frame's row,
+// rows wcap apart) or straight from the traces (rows fstride apart).  The
+// window bounds every in-range voxel's taps, so the clamp of the LDS index
+// never moves a tap; it only keeps reads inside the rows.
+template <int KIND, int V, int FB, bool FROM_LDS, typename S>
+__device__ __forceinline__ void line_update(const KArgs& a, const S* __restrict__ src, int s0,
+                                            int s_end, int wcap, long long fstride, int nf,
+                                            const double* px, const double* py, const double* pz,
+                                            const double* apF, const double* iapF,
+                                            const double* Bu, const double* Bp, const bool* act,
+                                            double exd, double eyd, double ezd, double uab,
+                                            double pab, const float* sc, float2 (*accT)[FB],
+                                            float* mT) {
+  constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    if (!act[v]) continue;
+    const Upd g = update_geom<LO, HI>(a, px[v], py[v], pz[v], apF[v], iapF[v], Bu[v], Bp[v], exd,
+                                      eyd, ezd, uab, pab);
+    mT[v] += g.w;   // .cu:301-304 (inside the aperture, before the range test)
+    if (!g.in_rng) continue;
+    float q[NT];
+    interp_weights<KIND>(g.d, q);
+    float sn, cs;
+    sincos2pi(g.p, sn, cs);
+    const float wr = g.w * cs, wi = g.w * sn;
+    const S* trv;
+    long long rs;
+    if constexpr (FROM_LDS) {
+      trv = src + min(max(g.m - LO - s0, 0), s_end - s0 - NT);
+      rs = wcap;
+    } else {
+      trv = src + (g.m - LO);
+      rs = fstride;
+    }
+#pragma unroll
+    for (int fi = 0; fi < FB; ++fi) {
+      if (fi < nf) {
+        const S* trf = trv + fi * rs;
+        float re = 0.0f, im = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+          const float2 smp = ld_sample(trf, k);
+          re = fmaf(q[k], smp.x, re);
+          im = fmaf(q[k], smp.y, im);
+        }
+        const float r2 = wr * sc[fi], i2 = wi * sc[fi];
+        accT[v][fi].x = fmaf(re, r2, fmaf(-im, i2, accT[v][fi].x));
+        accT[v][fi].y = fmaf(re, i2, fmaf(im, r2, accT[v][fi].y));
+      }
+    }
+  }
+}
+
 template <int KIND, int V, int FB, typename S>
 __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned by,
                                           unsigned char* smem_raw) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
-  S* tr = reinterpret_cast<S*>(smem_raw);
+  constexpr int E = 16 / sizeof(S);   // samples per 16-byte chunk
+  S* tr = reinterpret_cast<S*>(smem_raw);   // [FB][wcap]
   const S* iq = reinterpret_cast<const S*>(a.iq);
+  __shared__ float s_box[4][6];
+  bool ok;   // scratch for voxel_of at the stores
 
   const int tid = threadIdx.x;
-  const long long pblk = (long long)bx * (kThreads * V);
   const int f0 = by * FB;
   const int nf = min(FB, a.n_frame - f0);
   const int ns = a.n_sample;
+  const int wcap = a.wcap;
+  // chunk advance; at least one 16-byte chunk so the chunk loop always ends
+  // (rows shorter than NT + E only occur with traces too short for any tap)
+  const int step = max((wcap - NT + 1) / E * E, E);
   const long long fstride = (long long)a.n_tx * a.n_ch * ns;
 
   double px[V], py[V], pz[V], apF[V], iapF[V];
   bool valid[V];
   float2 acc[V][FB];
   float macc[V];
+  float bb[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
 #pragma unroll
   for (int v = 0; v < V; ++v) {
-    const long long p = pblk + v * kThreads + tid;
-    valid[v] = p < a.n_pix;
+    const long long p = voxel_of(a, bx, V, v, tid, valid[v]);
     const long long pc = valid[v] ? p : 0;
-    px[v] = a.gx[pc];
-    py[v] = a.gy[pc];
-    pz[v] = a.gz[pc];
+    const float fx = a.gx[pc], fy = a.gy[pc], fz = a.gz[pc];
+    px[v] = fx;
+    py[v] = fy;
+    pz[v] = fz;
     apF[v] = pz[v] / (double)a.fnum;   // .cu:174
     iapF[v] = (double)a.fnum / pz[v];
     macc[v] = 0.0f;
 #pragma unroll
     for (int fi = 0; fi < FB; ++fi) acc[v][fi] = make_float2(0.0f, 0.0f);
+    if (valid[v]) {
+      bb[0] = fminf(bb[0], fx); bb[1] = fmaxf(bb[1], fx);
+      bb[2] = fminf(bb[2], fy); bb[3] = fmaxf(bb[3], fy);
+      bb[4] = fminf(bb[4], fz); bb[5] = fmaxf(bb[5], fz);
+    }
   }
+  // bounding box of the workgroup's voxels (wave shuffles, then LDS)
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float y = __shfl_xor(bb[k], o, 64);
+      bb[k] = (k & 1) ? fmaxf(bb[k], y) : fminf(bb[k], y);
+    }
+  }
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s_box[tid >> 6][k] = bb[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    float r = s_box[0][k];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) r = (k & 1) ? fmaxf(r, s_box[w][k]) : fminf(r, s_box[w][k]);
+    bb[k] = uni(r);
+  }
+  // receive-axis, transmit-axis and depth ranges (held in scalar registers)
+  const double lat_lo = uni((double)(a.receive_x ? bb[0] : bb[2]));
+  const double lat_hi = uni((double)(a.receive_x ? bb[1] : bb[3]));
+  const double tx_lo = uni((double)(a.receive_x ? bb[2] : bb[0]));
+  const double tx_hi = uni((double)(a.receive_x ? bb[3] : bb[1]));
+  const double zlo = uni((double)bb[4]), zhi = uni((double)bb[5]);
+  const double apmax = uni(a.fnum > 0.0f ? zhi / (double)a.fnum : zlo / (double)a.fnum);
 
   for (int t = 0; t < a.n_tx; ++t) {
     const float vxf = a.vx[t], vyf = a.vy[t], vzf = a.vz[t];
@@ -388,15 +549,40 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
 #pragma unroll
       for (int fi = 0; fi < FB; ++fi) accT[v][fi] = make_float2(0.0f, 0.0f);
     }
+    // transmit sample offset over the box: Bu is increasing in the distance
+    double BuMin, BuMax;
+    {
+      double l0, l1, z0, z1;
+      abs_range(tx_lo - (double)(a.receive_x ? vyf : vxf), tx_hi - (double)(a.receive_x ? vyf : vxf), l0, l1);
+      abs_range(zlo - (double)vzf, zhi - (double)vzf, z0, z1);
+      BuMin = uni(fma(a.kfs, sqrt(l0 * l0 + z0 * z0) - d0, a.ut0));
+      BuMax = uni(fma(a.kfs, sqrt(l1 * l1 + z1 * z1) - d0, a.ut0));
+    }
 
     for (int c = 0; c < a.n_ch; ++c) {
-      __syncthreads();
-      stage<S>(tr, iq + ((long long)f0 * a.n_tx * a.n_ch + (long long)t * a.n_ch + c) * ns,
-               fstride, nf, ns, a.vec_ok);
-      __syncthreads();
       const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
       double uab, pab;
       ab_terms(a, t, c, uab, pab);
+      // sample window of this (tilt, line) over the box; lines whose aperture
+      // misses every voxel (min |lateral| >= max z/F) contribute nothing
+      double A0, A1, Z0, Z1;
+      abs_range(lat_lo - (a.receive_x ? exd : eyd), lat_hi - (a.receive_x ? exd : eyd), A0, A1);
+      abs_range(zlo - ezd, zhi - ezd, Z0, Z1);
+      const bool any_ap = uni((int)(A0 < apmax)) != 0;
+      int s0 = 0, w_hi = 0;
+      if (any_ap) {
+        A1 = fmin(A1, apmax);
+        const double umin = BuMin + a.kfs * sqrt(A0 * A0 + Z0 * Z0) + uab;
+        const double umax = BuMax + a.kfs * sqrt(A1 * A1 + Z1 * Z1) + uab;
+        // taps [floor(u) - LO, floor(u) + HI]; one sample of margin on each
+        // side covers rounding in the bounds
+        const int w_lo = (int)fmin(fmax(0.0, floor(umin) - (LO + 1)), (double)ns);
+        w_hi = (int)fmax(fmin((double)ns, floor(umax) + (HI + 2)), 0.0);
+        s0 = (w_lo / E) * E;
+      }
+      s0 = uni(s0);
+      w_hi = uni(w_hi);
+      const S* gtr = iq + ((long long)f0 * a.n_tx * a.n_ch + (long long)t * a.n_ch + c) * ns;
       float sc[FB];
 #pragma unroll
       for (int fi = 0; fi < FB; ++fi) {
@@ -411,45 +597,63 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
 #pragma unroll
           for (int fi = 0; fi < FB; ++fi) accT[v][fi] = make_float2(0.0f, 0.0f);
       }
+      // The window [s0, w_hi) is staged in chunks of at most wcap samples: one
+      // chunk unless the window is longer than the LDS rows.  Chunk [cb, ce)
+      // takes the voxels whose first tap lies in [cb, cb + step) (the last
+      // chunk: from cb on), and it holds all of their taps.
+      int cb = s0;
+      if (any_ap) do {
+        const int ce = min(cb + wcap, w_hi);
+        const bool first = cb == s0, last = ce == w_hi;
+        __syncthreads();
+        stage_window<S>(tr, gtr, fstride, nf, cb, ce - cb, wcap, a.vec_ok);
+        __syncthreads();
+        const int b_hi = last ? ns : cb + step;
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        if (!act[v]) continue;
-        const Upd g = update_geom<LO, HI>(a, px[v], py[v], pz[v], apF[v], iapF[v], Bu[v], Bp[v],
-                                          exd, eyd, ezd, uab, pab);
-        mT[v] += g.w;   // .cu:301-304 (inside the aperture, before the range test)
-        if (!g.in_rng) continue;
-        float q[NT];
-        interp_weights<KIND>(g.d, q);
-        float sn, cs;
-        sincos2pi(g.p, sn, cs);
-        const float wr = g.w * cs, wi = g.w * sn;
-        const int base = g.m - LO;
+        for (int v = 0; v < V; ++v) {
+          if (!act[v]) continue;
+          const Upd g = update_geom<LO, HI>(a, px[v], py[v], pz[v], apF[v], iapF[v], Bu[v], Bp[v],
+                                            exd, eyd, ezd, uab, pab);
+          if (first) mT[v] += g.w;   // .cu:301-304 (inside the aperture, before the range test)
+          if (!g.in_rng) continue;
+          const int base = g.m - LO;
+          if ((!first && base < cb) || base >= b_hi) continue;
+          float q[NT];
+          interp_weights<KIND>(g.d, q);
+          float sn, cs;
+          sincos2pi(g.p, sn, cs);
+          const float wr = g.w * cs, wi = g.w * sn;
+          // the bounds put every tap inside the chunk; the clamp only keeps
+          // LDS reads inside the rows
+          const S* trv = tr + min(max(base - cb, 0), wcap - NT);
 #pragma unroll
-        for (int fi = 0; fi < FB; ++fi) {
-          if (fi < nf) {
-            const S* trf = tr + fi * ns + base;
-            float re = 0.0f, im = 0.0f;
+          for (int fi = 0; fi < FB; ++fi) {
+            if (fi < nf) {
+              const S* trf = trv + fi * wcap;
+              float re = 0.0f, im = 0.0f;
 #pragma unroll
-            for (int k = 0; k < NT; ++k) {
-              const float2 s = ld_sample(trf, k);
-              re = fmaf(q[k], s.x, re);
-              im = fmaf(q[k], s.y, im);
+              for (int k = 0; k < NT; ++k) {
+                const float2 smp = ld_sample(trf, k);
+                re = fmaf(q[k], smp.x, re);
+                im = fmaf(q[k], smp.y, im);
+              }
+              const float r2 = wr * sc[fi], i2 = wi * sc[fi];
+              accT[v][fi].x = fmaf(re, r2, fmaf(-im, i2, accT[v][fi].x));
+              accT[v][fi].y = fmaf(re, i2, fmaf(im, r2, accT[v][fi].y));
             }
-            const float r2 = wr * sc[fi], i2 = wi * sc[fi];
-            accT[v][fi].x = fmaf(re, r2, fmaf(-im, i2, accT[v][fi].x));
-            accT[v][fi].y = fmaf(re, i2, fmaf(im, r2, accT[v][fi].y));
           }
         }
-      }
+        cb += step;
+      } while (cb < w_hi);
       if (a.nosum) {   // NoSum.cu:228-229,275-278,376-380: out[c][f][t][p]
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           if (!valid[v]) continue;
-          const long long p = pblk + v * kThreads + tid;
 #pragma unroll
           for (int fi = 0; fi < FB; ++fi) {
             if (fi < nf) {
-              float2* o = a.out + (((long long)c * a.n_frame + f0 + fi) * a.n_tx + t) * a.n_pix + p;
+              float2* o = a.out + (((long long)c * a.n_frame + f0 + fi) * a.n_tx + t) * a.n_pix +
+                          voxel_of(a, bx, V, v, tid, ok);
               float2 r = accT[v][fi];
               if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
               *o = r;
@@ -473,18 +677,18 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         if (!valid[v]) continue;
-        const long long p = pblk + v * kThreads + tid;
 #pragma unroll
         for (int fi = 0; fi < FB; ++fi) {
           if (fi < nf && !a.nosum) {
-            float2* o = a.out + ((long long)(f0 + fi) * a.n_tx + t) * a.n_pix + p;
+            float2* o = a.out + ((long long)(f0 + fi) * a.n_tx + t) * a.n_pix +
+                        voxel_of(a, bx, V, v, tid, ok);
             float2 r = accT[v][fi];
             if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
             *o = r;
           }
         }
         if (a.mask && by == 0) {
-          float* mk = a.mask + (long long)t * a.n_pix + p;
+          float* mk = a.mask + (long long)t * a.n_pix + voxel_of(a, bx, V, v, tid, ok);
           *mk = a.accumulate ? *mk + mT[v] : mT[v];
         }
       }
@@ -495,22 +699,22 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       if (!valid[v]) continue;
-      const long long p = pblk + v * kThreads + tid;
 #pragma unroll
       for (int fi = 0; fi < FB; ++fi) {
         if (fi < nf) {
-          float2* o = a.out + (long long)(f0 + fi) * a.n_pix + p;
+          float2* o = a.out + (long long)(f0 + fi) * a.n_pix + voxel_of(a, bx, V, v, tid, ok);
           float2 r = acc[v][fi];
           if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
           *o = r;
         }
       }
       if (a.mask && by == 0) {
-        float* mk = a.mask + p;
+        float* mk = a.mask + voxel_of(a, bx, V, v, tid, ok);
         *mk = a.accumulate ? *mk + macc[v] : macc[v];
       }
     }
   }
+  __syncthreads();   // s_box and the LDS rows may be reused by the caller's next block
 }
 
 template <int KIND, int V, int FB, typename S>
@@ -677,15 +881,30 @@ KArgs make_args(const rca_das_desc* d, int n_frame) {
   a.n_pix = d->n_pix; a.n_ch = d->n_channel; a.n_tx = d->n_tx; a.n_sample = d->n_sample;
   a.n_frame = n_frame;
   a.receive_x = d->receive_x != 0; a.mask_tx = d->mask_transmit != 0;
+  a.depth_run = d->depth_run >= 64 ? d->depth_run : 0;
   return a;
+}
+
+// LDS samples per frame row: the whole trace (rounded up to 16 bytes) unless
+// workgroups are depth tiles, whose windows are short; then rows are capped
+// so FB rows take at most kWinLds bytes.
+constexpr int kWinLds = 32 * 1024;
+int pick_wcap(const KArgs& a, int fb, size_t sbytes) {
+  const int e = (int)(16 / sbytes);
+  const int full = (a.n_sample + e - 1) / e * e;
+  if (!a.depth_run) return full;
+  const int cap = (int)(kWinLds / ((size_t)fb * sbytes)) / e * e;
+  return cap < full ? cap : full;
 }
 
 template <typename S> struct SName;
 
 template <int KIND, int V, int FB, typename S>
 int launch_fwd_t(KArgs a, hipStream_t st) {
-  const size_t lds = (size_t)FB * a.n_sample * sizeof(S);
-  const unsigned gx = (unsigned)(((long long)a.n_pix + kThreads * V - 1) / (kThreads * V));
+  a.wcap = pick_wcap(a, FB, sizeof(S));
+  const size_t lds = (size_t)FB * a.wcap * sizeof(S);
+  if (lds > (size_t)kMaxLds) return fail(RCA_ERR_UNSUPPORTED, "FB=%d traces do not fit in LDS", FB);
+  const unsigned gx = (unsigned)fwd_blocks(a.n_pix, a.depth_run, V);
   const unsigned gy = (unsigned)((a.n_frame + FB - 1) / FB);
   if (gx == 0 || gy == 0) return RCA_OK;
   hipLaunchKernelGGL((das_fwd<KIND, V, FB, S>), dim3(gx, gy, 1), dim3(kThreads), lds, st, a);
@@ -715,16 +934,17 @@ int launch_fwd_s(const KArgs& a, int interp, int V, int FB, hipStream_t st) {
   return fail(RCA_ERR_INVALID, "unknown interpolation %d", interp);
 }
 
-int pick_fb(int n_frame, int n_sample, size_t bytes) {
+int pick_fb(int n_frame, int n_sample, size_t bytes, bool tiled) {
   int fb = 8;
-  while (fb > 1 && (fb > n_frame || (size_t)fb * n_sample * bytes > (size_t)kMaxLds)) fb >>= 1;
+  // depth tiles stage short windows, so any FB fits; whole traces must fit
+  while (fb > 1 && (fb > n_frame || (!tiled && (size_t)fb * n_sample * bytes > (size_t)kMaxLds))) fb >>= 1;
   return fb;
 }
 
-int pick_v(long long n_pix, int n_fblocks, int fb) {
+int pick_v(long long n_pix, int depth_run, int n_fblocks, int fb) {
   // V = 2 keeps the kernel at 3 waves per SIMD (measured fastest); V = 1 when
   // that would leave fewer than ~4 workgroups per CU (256 CUs on MI355X).
-  const long long blocks2 = ((n_pix + kThreads * 2 - 1) / (kThreads * 2)) * n_fblocks;
+  const long long blocks2 = fwd_blocks(n_pix, depth_run, 2) * n_fblocks;
   return (fb <= 4 && blocks2 >= 1024) ? 2 : 1;
 }
 
@@ -799,11 +1019,9 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
   a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
   a.nosum = (flags & RCA_FLAG_NOSUM) != 0;
   a.vec_ok = (((uintptr_t)iq & 15) == 0) && (((size_t)d->n_sample * sbytes) % 16 == 0);
-  int fb = g_tune_fb ? g_tune_fb : pick_fb(n_frame, d->n_sample, sbytes);
-  if ((size_t)fb * d->n_sample * sbytes > (size_t)kMaxLds)
-    return fail(RCA_ERR_UNSUPPORTED, "FB=%d does not fit in LDS", fb);
+  int fb = g_tune_fb ? g_tune_fb : pick_fb(n_frame, d->n_sample, sbytes, a.depth_run != 0);
   const int nfb = (n_frame + fb - 1) / fb;
-  int v = g_tune_v ? g_tune_v : pick_v(d->n_pix, nfb, fb);
+  int v = g_tune_v ? g_tune_v : pick_v(d->n_pix, a.depth_run, nfb, fb);
   if ((v == 4 && fb > 1) || (v == 2 && fb > 4)) v = 1;   // instantiated shapes
   if (iq_dtype == RCA_IQ_C64) return launch_fwd_s<float2>(a, d->interp, v, fb, stream);
   return launch_fwd_s<__half2>(a, d->interp, v, fb, stream);
@@ -827,7 +1045,7 @@ int rca_adjoint(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_
   a.gin = reinterpret_cast<float2*>(grad_iq);
   a.gout_tx = grad_tx;
   a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
-  const int fb = pick_fb(n_frame, d->n_sample, 8);
+  const int fb = pick_fb(n_frame, d->n_sample, 8, false);
   const int nfb = (n_frame + fb - 1) / fb;
   const long long traces = (long long)d->n_channel * d->n_tx * nfb;
   int split = g_tune_split;
@@ -927,13 +1145,16 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
                   E.frame_begin, E.frame_end, E.n_frame);
     if (na + E.n_half > max_halves) return fail(RCA_ERR_UNSUPPORTED, "batch exceeds %d halves", max_halves);
     const int first = na;
-    int n_pix = -1;
+    int n_pix = -1, depth_run = 0;
     for (int h = 0; h < E.n_half; ++h) {
       const rca_das_desc* d = E.half[h];
       int rc = check_desc(d);
       if (rc) return rc;
-      if (n_pix >= 0 && d->n_pix != n_pix) return fail(RCA_ERR_INVALID, "entry %d: halves differ in nPix", q);
+      const int dr = d->depth_run >= 64 ? d->depth_run : 0;
+      if (n_pix >= 0 && (d->n_pix != n_pix || dr != depth_run))
+        return fail(RCA_ERR_INVALID, "entry %d: halves differ in nPix or depth_run", q);
       n_pix = d->n_pix;
+      depth_run = dr;
       if (d->n_sample > n_sample_max)
         return fail(RCA_ERR_UNSUPPORTED, "entry %d: nSample %d > batch maximum %d", q, d->n_sample, n_sample_max);
       if (!E.iq[h] || (E.iq_dtype == RCA_IQ_F16X2 && !E.trace_scale[h]))
@@ -948,15 +1169,16 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
       a.nosum = 0;
       const size_t sb = E.iq_dtype == RCA_IQ_C64 ? 8 : 4;
       a.vec_ok = (((uintptr_t)E.iq[h] & 15) == 0) && (((size_t)d->n_sample * sb) % 16 == 0);
+      a.wcap = (int)((d->n_sample + 16 / sb - 1) / (16 / sb) * (16 / sb));   // whole trace
       args[na].k = a;
       args[na].iq_dtype = E.iq_dtype;
       ++na;
     }
-    const int nvb = (n_pix + kThreads * kBatchV - 1) / (kThreads * kBatchV);
+    const long long nvb = fwd_blocks(n_pix, depth_run, kBatchV);
     for (int fbk = E.frame_begin / fb; fbk * fb < E.frame_end; ++fbk) {
-      for (int vb = 0; vb < nvb; ++vb) {
+      for (long long vb = 0; vb < nvb; ++vb) {
         if (ni >= max_items) return fail(RCA_ERR_UNSUPPORTED, "batch exceeds %d items", max_items);
-        items[ni++] = make_int4(first, E.n_half, vb, fbk);
+        items[ni++] = make_int4(first, E.n_half, (int)vb, fbk);
       }
     }
   }
@@ -970,7 +1192,9 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_bl
   g_err.clear();
   if (!dev_ws || max_halves < 1 || grid_blocks < 1 || n_sample_max < 1)
     return fail(RCA_ERR_INVALID, "bad batch launch arguments");
-  const size_t lds = (size_t)n_sample_max * 8;
+  // rows of whole traces: FP32 FB=1 x 8 B or FP16 FB=2 x 4 B per sample,
+  // each row rounded up to 16 bytes
+  const size_t lds = ((size_t)n_sample_max + 3) / 4 * 4 * 8;
   if (lds > (size_t)kMaxLds) return fail(RCA_ERR_UNSUPPORTED, "nSample %d exceeds LDS", n_sample_max);
   const unsigned char* ws = reinterpret_cast<const unsigned char*>(dev_ws);
   const int* hdr = reinterpret_cast<const int*>(ws);

@@ -65,6 +65,13 @@ def prepare_geometry(Geometry: dict, Transmit: dict) -> dict:
             "aperture": [float(np.float32(a)) for a in ap]}
 
 
+def _depth_run(g) -> int:
+    """Length of a grid array's first axis (its fastest in column-major order),
+    0 for a flat voxel list."""
+    shape = tuple(g.shape) if hasattr(g, "shape") else np.shape(g)
+    return int(shape[0]) if len(shape) >= 2 else 0
+
+
 def _dev_f32(a, device):
     if isinstance(a, torch.Tensor):
         t = a.detach().to(device=device, dtype=torch.float32)
@@ -102,7 +109,7 @@ class RcaPlan:
 
     def __init__(self, *, grid, ele, vs, min_dist, receive_x, aperture, fs, fd, c, t0, fnumber,
                  n_sample, interp="quadratic", mask_transmit=False, ab_law=None,
-                 ab_law_mode=None, device=None):
+                 ab_law_mode=None, depth_run=0, device=None):
         self.lib = _lib.load()
         self.device = torch.device(device) if device is not None else torch.device(
             "cuda", torch.cuda.current_device())
@@ -154,12 +161,20 @@ class RcaPlan:
         d.speed_of_sound = float(c)
         d.start_time = float(t0)
         d.fnumber = float(fnumber)
+        # grid stored depth-fastest with this many depths: workgroups take depth
+        # tiles (a speed hint only; the results do not depend on it)
+        self.depth_run = int(depth_run) if depth_run and n_pix % int(depth_run) == 0 else 0
+        d.depth_run = self.depth_run
         self.desc = d
 
     # ------------------------------------------------------------------
     @classmethod
-    def from_reference(cls, Geometry, Transmit, Param, Options=None, n_sample=None, device=None):
-        """Plan from the reference's Geometry/Transmit/Param/Options dicts."""
+    def from_reference(cls, Geometry, Transmit, Param, Options=None, n_sample=None, device=None,
+                       depth_run=None):
+        """Plan from the reference's Geometry/Transmit/Param/Options dicts.
+        ``depth_run`` is the length of the grid's first (fastest) axis, Nz for
+        the reference's [Nz, Nx, Ny] meshgrids; by default it is read from the
+        shape of ``Geometry['gridBfZ']`` when that is at least 2-D."""
         Options = dict(Options or {})
         prep = prepare_geometry(Geometry, Transmit)
         grid = [np.asarray(Geometry[k], dtype=np.float32) if not isinstance(Geometry[k], torch.Tensor)
@@ -171,7 +186,9 @@ class RcaPlan:
                    fnumber=Param.get("fnumber", 1.0), n_sample=n_sample,
                    interp=interpolation_name(Options),
                    mask_transmit=bool(Options.get("flagMaskTransmit", True)),
-                   ab_law=Param.get("AbLawRx"), ab_law_mode=Param.get("AbLawMode"), device=device)
+                   ab_law=Param.get("AbLawRx"), ab_law_mode=Param.get("AbLawMode"),
+                   depth_run=_depth_run(Geometry["gridBfZ"]) if depth_run is None else depth_run,
+                   device=device)
 
     # ------------------------------------------------------------------
     def _stream(self):

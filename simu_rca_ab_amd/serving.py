@@ -222,6 +222,7 @@ class GpuBackend:
             d.aperture_x, d.aperture_y, d.aperture_z = tab["aperture"]
             d.sampling_frequency, d.demodulation_frequency = self.fs, self.fd
             d.speed_of_sound, d.start_time, d.fnumber = self.c, self.t0, self.fnumber
+            d.depth_run = spec.nz   # voxels are stored depth-fastest
             self._descs[key] = d
         return self._descs[key]
 

@@ -1,0 +1,130 @@
+"""GPU parity of the depth-tiled forward launch (grid hint ``depth_run``).
+
+With the hint, a workgroup owns 64 depths x 4V columns and stages only the
+sample window that its voxels' taps can reach on each (tilt, line).  Its
+results must match the untiled launch (1e-6 relative: the per-voxel arithmetic
+is the same; only the compiler's contraction choices can differ between
+instantiations) and the FP64 oracle (1e-5, 0.1 dB).  Cases: every
+interpolation, both halves, per-tilt and compounded output, frame counts that
+leave partial frame blocks, column counts that leave partial column groups, a
+hint that does not divide the voxel count, a depth-coarse grid whose windows
+are longer than the LDS rows (staged in several chunks), FP16-I/Q and NoSum.
+"""
+import numpy as np
+import pytest
+import torch
+
+from simu_rca_ab_amd import _lib, configs as C
+from simu_rca_ab_amd.plan import RcaPlan, quantize_fp16
+from tests import helpers as H
+from tests.test_gpu_forward import check, oracle_fwd, say
+
+pytestmark = pytest.mark.gpu
+
+TILE = C.scaled(C.get("cfg1"), nx=5, ny=3, nz=128, n_scat=300)   # 15 columns, 2 depth tiles
+
+
+def plans(cfg, half_index, dev, interp=None, G=None, depth_run=None):
+    G0, T, P, Op = H.case(cfg, half_index)
+    G = G or G0
+    if interp:
+        Op = dict(Op, interpolation=interp)
+    Gf = H.flat_geometry(G)
+    kw = dict(n_sample=cfg.n_sample, device=dev)
+    flat = RcaPlan.from_reference(Gf, T, P, Op, depth_run=0, **kw)
+    tiled = RcaPlan.from_reference(Gf, T, P, Op, depth_run=depth_run or cfg.nz, **kw)
+    assert flat.depth_run == 0 and tiled.depth_run == (depth_run or cfg.nz)
+    return flat, tiled, G
+
+
+@pytest.mark.parametrize("interp", ["quadratic", "lanczos", "lanczos3", "lanczos4"])
+@pytest.mark.parametrize("half_index", [0, 1])
+def test_tiled_matches_untiled_and_oracle(dev, interp, half_index):
+    say(f"case tiled 5x3x128 interp={interp} half={half_index}, 3 frames, per tilt + compound")
+    iq = H.iq_frame(TILE, half_index, n_frame=3)
+    flat, tiled, G = plans(TILE, half_index, dev, interp)
+    x = iq.to(dev)
+    for compound in (False, True):
+        o_f, m_f = flat.forward(x, compound=compound)
+        o_t, m_t = tiled.forward(x, compound=compound)
+        torch.cuda.synchronize()
+        r = H.rel_l2(o_t.cpu().numpy(), o_f.cpu().numpy())
+        say(f"tiled vs untiled compound={compound}: rel-L2 {r:.3e}")
+        assert r <= 1e-6
+        assert H.rel_l2(m_t.cpu().numpy(), m_f.cpu().numpy()) <= 1e-6
+    ref, rmask = oracle_fwd(TILE, half_index, iq, interp)
+    check(o_t.cpu().numpy(), ref.sum(axis=1, keepdims=True), f"tiled {interp}/{half_index} vs oracle")
+    assert H.rel_l2(m_t.cpu().numpy(), rmask.sum(axis=0, keepdims=True)) <= 1e-5
+
+
+def test_tiled_launch_shapes(dev):
+    say("case tiled V x FB shapes, 5 frames")
+    iq = H.iq_frame(TILE, 0, n_frame=5).to(dev)
+    flat, tiled, G = plans(TILE, 0, dev, "lanczos4")
+    base = flat.forward(iq)[0].cpu().numpy()
+    try:
+        for v, fb in ((1, 1), (2, 1), (4, 1), (1, 2), (2, 2), (1, 4), (2, 4), (1, 8)):
+            _lib.set_tuning(v, fb, 0)
+            o = tiled.forward(iq)[0].cpu().numpy()
+            r = H.rel_l2(o, base)
+            say(f"V={v} FB={fb}: rel-L2 vs untiled {r:.3e}")
+            assert r <= 1e-6, (v, fb)
+    finally:
+        _lib.set_tuning(0, 0, 0)
+
+
+def test_hint_not_dividing_voxels(dev):
+    say("case depth_run=100 on a 128-deep grid (partial last column, mixed tiles)")
+    iq = H.iq_frame(TILE, 1, n_frame=2).to(dev)
+    flat, tiled, G = plans(TILE, 1, dev, "lanczos3")
+    tiled.desc.depth_run = 100     # the plan keeps only dividing hints; the C ABI takes any
+    o_f, m_f = flat.forward(iq, compound=False)
+    o_t, m_t = tiled.forward(iq, compound=False)
+    torch.cuda.synchronize()
+    assert H.rel_l2(o_t.cpu().numpy(), o_f.cpu().numpy()) <= 1e-6
+    assert H.rel_l2(m_t.cpu().numpy(), m_f.cpu().numpy()) <= 1e-6
+
+
+def test_windows_longer_than_rows(dev):
+    # depths 0.6 mm apart: a 64-depth tile spans 38 mm, about 750 samples per
+    # window, while FB = 8 FP32 rows hold 512 samples (two chunks)
+    cfg = C.scaled(TILE, nx=4, ny=3)
+    x, y, _ = C.grid_axes(cfg)
+    z = 5e-3 + np.arange(cfg.nz) * 0.6e-3
+    G0, T, P, Op = H.case(cfg, 0)
+    G = C.geometry(cfg, cfg.halves[0], C.meshgrid_zxy(x, y, z))
+    say("case depth-coarse grid (windows staged in several chunks), 8 frames, FB=8")
+    iq = H.iq_frame(cfg, 0, n_frame=8)
+    flat, tiled, _ = plans(cfg, 0, dev, "lanczos4", G=G)
+    x8 = iq.to(dev)
+    base = flat.forward(x8)[0].cpu().numpy()
+    try:
+        for v, fb in ((1, 8), (2, 4), (2, 1)):
+            _lib.set_tuning(v, fb, 0)
+            o = tiled.forward(x8)[0].cpu().numpy()
+            r = H.rel_l2(o, base)
+            say(f"V={v} FB={fb}: rel-L2 vs untiled {r:.3e}")
+            assert r <= 1e-6, (v, fb)
+    finally:
+        _lib.set_tuning(0, 0, 0)
+    ref, _ = oracle_fwd(cfg, 0, iq, "lanczos4", G=G)
+    check(base, ref.sum(axis=1, keepdims=True), "depth-coarse vs oracle")
+
+
+def test_tiled_fp16_and_nosum(dev):
+    say("case tiled FP16-I/Q (4 frames) and NoSum")
+    iq = H.iq_frame(TILE, 0, n_frame=4).to(dev)
+    flat, tiled, G = plans(TILE, 0, dev, "lanczos4")
+    h, s = quantize_fp16(iq)
+    o_f = flat.forward(h, s)[0]
+    o_t = tiled.forward(h, s)[0]
+    torch.cuda.synchronize()
+    r = H.rel_l2(o_t.cpu().numpy(), o_f.cpu().numpy())
+    say(f"fp16 tiled vs untiled: rel-L2 {r:.3e}")
+    assert r <= 1e-6
+    n_f = flat.forward(iq[:1], compound=False, nosum=True, want_mask=False)[0]
+    n_t = tiled.forward(iq[:1], compound=False, nosum=True, want_mask=False)[0]
+    torch.cuda.synchronize()
+    r = H.rel_l2(n_t.cpu().numpy(), n_f.cpu().numpy())
+    say(f"nosum tiled vs untiled: rel-L2 {r:.3e}")
+    assert r <= 1e-6
