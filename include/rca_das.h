@@ -171,9 +171,11 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_bl
 /*
  * Launch-shape override for tuning sweeps (no reference counterpart):
  * voxels per lane v in {1,2,4}, frames per workgroup fb in {1,2,4,8}, adjoint
- * voxel splits; 0 restores the automatic choice.  Process-wide.
+ * voxel splits, LDS budget of a forward workgroup in KiB (16..64, which sets
+ * how many tilts share one pass over the receive lines); 0 restores the
+ * automatic choice.  Process-wide.
  */
-int rca_set_tuning(int v, int fb, int split);
+int rca_set_tuning(int v, int fb, int split, int lds_kib);
 
 /* Last error message of the calling thread ("" when none). */
 const char* rca_last_error(void);

@@ -57,7 +57,7 @@ def load():
     dp = ctypes.POINTER(RcaDasDesc)
     lib.rca_forward.argtypes = [dp, vp, i, vp, i, vp, vp, i, vp]
     lib.rca_adjoint.argtypes = [dp, vp, i, i, vp, i, vp]
-    lib.rca_set_tuning.argtypes = [i, i, i]
+    lib.rca_set_tuning.argtypes = [i, i, i, i]
     lib.bfVirtualSourceGeneric_RCA.argtypes = [vp] * 13 + [i, f, f, f, f, f, f, f, f] + [i] * 7 + [vp]
     for name in ("rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA"):
         getattr(lib, name).restype = ctypes.c_int
@@ -76,6 +76,6 @@ def version() -> str:
     return load().rca_version().decode()
 
 
-def set_tuning(v: int = 0, fb: int = 0, split: int = 0):
+def set_tuning(v: int = 0, fb: int = 0, split: int = 0, lds_kib: int = 0):
     """Override the automatic launch shape (0 = automatic)."""
-    check(load().rca_set_tuning(int(v), int(fb), int(split)))
+    check(load().rca_set_tuning(int(v), int(fb), int(split), int(lds_kib)))

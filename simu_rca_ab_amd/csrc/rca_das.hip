@@ -79,7 +79,8 @@ struct KArgs {
   int n_pix, n_ch, n_tx, n_sample, n_frame;
   int receive_x, mask_tx, ab_mode, compound, accumulate, vec_ok, nosum;
   int gout_tx, n_split, atomic_flush;
-  int depth_run, wcap;   // voxel tiling hint; LDS samples per frame row
+  int depth_run, wcap;   // voxel tiling hint; LDS samples per row
+  int tc;                // tilts per chunk of the forward kernel
 };
 
 // ----------------------------------------------------------------------------
@@ -227,44 +228,35 @@ __device__ __forceinline__ void interp_weights(float d, float* q) {
 }
 
 // Transmit insonification cone (bfVirtualSourceGeneric_RCA.cu:85-98, 191-214),
-// in FP64.
-__device__ __forceinline__ double angle_ap(float ax, float ay, float bx, float by, float cx,
-                                           float cy) {
+// in FP64.  The reference compares angles, acos(cos_p) <= acos(cos_cone); acos
+// is decreasing on [-1, 1] and NaN outside it, so the same test reads
+// cos_p in [-1, 1], cos_cone in [-1, 1] and cos_p >= cos_cone, without acos.
+__device__ __forceinline__ double cos_ap(float ax, float ay, float bx, float by, float cx,
+                                         float cy) {
   const double abx = (double)bx - ax, aby = (double)by - ay;
   const double acx = (double)cx - ax, acy = (double)cy - ay;
-  return acos((abx * acx + aby * acy) / sqrt((abx * abx + aby * aby) * (acx * acx + acy * acy)));
+  return (abx * acx + aby * acy) / sqrt((abx * abx + aby * aby) * (acx * acx + acy * acy));
+}
+
+__device__ __forceinline__ bool within_cone(double c, double cone) {
+  return c >= -1.0 && c <= 1.0 && cone >= -1.0 && cone <= 1.0 && c >= cone;
 }
 
 __device__ __forceinline__ bool tx_mask(const KArgs& a, float vx, float vy, float vz, float px,
                                         float py, float pz) {
   const float hx = 0.5f * a.apx, hy = 0.5f * a.apy, az = a.apz;
-  const double coneX = angle_ap(vx, vz, -hx, -az, hx, -az);
-  const double coneY = angle_ap(vy, vz, -hy, -az, hy, -az);
-  return angle_ap(vx, vz, -hx, -az, px, pz) <= coneX && angle_ap(vx, vz, hx, -az, px, pz) <= coneX &&
-         angle_ap(vy, vz, -hy, -az, py, pz) <= coneY && angle_ap(vy, vz, hy, -az, py, pz) <= coneY;
+  const double coneX = cos_ap(vx, vz, -hx, -az, hx, -az);
+  const double coneY = cos_ap(vy, vz, -hy, -az, hy, -az);
+  return within_cone(cos_ap(vx, vz, -hx, -az, px, pz), coneX) &&
+         within_cone(cos_ap(vx, vz, hx, -az, px, pz), coneX) &&
+         within_cone(cos_ap(vy, vz, -hy, -az, py, pz), coneY) &&
+         within_cone(cos_ap(vy, vz, hy, -az, py, pz), coneY);
 }
 
 // Sample access: complex64 or half2 (the per-trace scale is applied to the
 // rotation weights by the caller).
 __device__ __forceinline__ float2 ld_sample(const float2* s, int i) { return s[i]; }
 __device__ __forceinline__ float2 ld_sample(const __half2* s, int i) { return __half22float2(s[i]); }
-
-// Cooperative copy of nf traces (frame stride fstride samples) into LDS.
-template <typename S>
-__device__ __forceinline__ void stage(S* __restrict__ dst, const S* __restrict__ src,
-                                      long long fstride, int nf, int ns, bool vec) {
-  if (vec) {
-    const int n16 = (int)((long long)ns * sizeof(S) / 16);
-    for (int fi = 0; fi < nf; ++fi) {
-      const uint4* s = reinterpret_cast<const uint4*>(src + fi * fstride);
-      uint4* d = reinterpret_cast<uint4*>(dst + (long long)fi * ns);
-      for (int i = threadIdx.x; i < n16; i += kThreads) d[i] = s[i];
-    }
-  } else {
-    for (int fi = 0; fi < nf; ++fi)
-      for (int i = threadIdx.x; i < ns; i += kThreads) dst[fi * ns + i] = src[fi * fstride + i];
-  }
-}
 
 // Geometry of one (voxel, tilt, line) update shared by forward and adjoint.
 struct Upd {
@@ -380,124 +372,106 @@ __device__ __forceinline__ double uni(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Lane j's value of x, in scalar registers (j a compile-time constant).
+__device__ __forceinline__ float lane_val(float x, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+}
+__device__ __forceinline__ double lane_val(double x, int j) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), j);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // |t| over t in [lo, hi]: smallest and largest.
 __device__ __forceinline__ void abs_range(double lo, double hi, double& amin, double& amax) {
   amin = (lo <= 0.0 && hi >= 0.0) ? 0.0 : fmin(fabs(lo), fabs(hi));
   amax = fmax(fabs(lo), fabs(hi));
 }
 
-// Copy samples [s0, s0 + len) of the nf traces of one (tilt, line) into LDS
-// rows of wcap samples.
-template <typename S>
-__device__ __forceinline__ void stage_window(S* __restrict__ dst, const S* __restrict__ src,
-                                             long long fstride, int nf, int s0, int len, int wcap,
-                                             bool vec) {
-  if (vec) {   // s0 and the row length are multiples of 16 bytes
-    const int n16 = (int)(((long long)len * sizeof(S) + 15) / 16);
-    for (int fi = 0; fi < nf; ++fi) {
-      const uint4* s = reinterpret_cast<const uint4*>(src + fi * fstride + s0);
-      uint4* d = reinterpret_cast<uint4*>(dst + (long long)fi * wcap);
-      for (int i = threadIdx.x; i < n16; i += kThreads) d[i] = s[i];
-    }
-  } else {
-    for (int fi = 0; fi < nf; ++fi)
-      for (int i = threadIdx.x; i < len; i += kThreads) dst[fi * wcap + i] = src[fi * fstride + s0 + i];
-  }
+// Transmit bounds of one tilt over a box (transmit-axis range [l_lo, l_hi],
+// depth range [z_lo, z_hi]): the sample offset Bu = fs*((|P-VS| - d0)/c + t0)
+// grows with the distance, so its extremes come from the extreme distances.
+__device__ __forceinline__ void tx_bounds(const KArgs& a, double l_lo, double l_hi, double z_lo,
+                                          double z_hi, float vl, float vz, double d0, float& bmin,
+                                          float& bmax) {
+  double l0, l1, z0, z1;
+  abs_range(l_lo - (double)vl, l_hi - (double)vl, l0, l1);
+  abs_range(z_lo - (double)vz, z_hi - (double)vz, z0, z1);
+  bmin = (float)fma(a.kfs, sqrt(l0 * l0 + z0 * z0) - d0, a.ut0);
+  bmax = (float)fma(a.kfs, sqrt(l1 * l1 + z1 * z1) - d0, a.ut0);
 }
 
-// Adds one (tilt, line) to the partial sums of a lane's V voxels.  Taps come
-// from a staged window (FROM_LDS: samples [s0, s_end) of each frame's row,
-// rows wcap apart) or straight from the traces (rows fstride apart).  The
-// window bounds every in-range voxel's taps, so the clamp of the LDS index
-// never moves a tap; it only keeps reads inside the rows.
-template <int KIND, int V, int FB, bool FROM_LDS, typename S>
-__device__ __forceinline__ void line_update(const KArgs& a, const S* __restrict__ src, int s0,
-                                            int s_end, int wcap, long long fstride, int nf,
-                                            const double* px, const double* py, const double* pz,
-                                            const double* apF, const double* iapF,
-                                            const double* Bu, const double* Bp, const bool* act,
-                                            double exd, double eyd, double ezd, double uab,
-                                            double pab, const float* sc, float2 (*accT)[FB],
-                                            float* mT) {
-  constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
-#pragma unroll
-  for (int v = 0; v < V; ++v) {
-    if (!act[v]) continue;
-    const Upd g = update_geom<LO, HI>(a, px[v], py[v], pz[v], apF[v], iapF[v], Bu[v], Bp[v], exd,
-                                      eyd, ezd, uab, pab);
-    mT[v] += g.w;   // .cu:301-304 (inside the aperture, before the range test)
-    if (!g.in_rng) continue;
-    float q[NT];
-    interp_weights<KIND>(g.d, q);
-    float sn, cs;
-    sincos2pi(g.p, sn, cs);
-    const float wr = g.w * cs, wi = g.w * sn;
-    const S* trv;
-    long long rs;
-    if constexpr (FROM_LDS) {
-      trv = src + min(max(g.m - LO - s0, 0), s_end - s0 - NT);
-      rs = wcap;
-    } else {
-      trv = src + (g.m - LO);
-      rs = fstride;
-    }
-#pragma unroll
-    for (int fi = 0; fi < FB; ++fi) {
-      if (fi < nf) {
-        const S* trf = trv + fi * rs;
-        float re = 0.0f, im = 0.0f;
-#pragma unroll
-        for (int k = 0; k < NT; ++k) {
-          const float2 smp = ld_sample(trf, k);
-          re = fmaf(q[k], smp.x, re);
-          im = fmaf(q[k], smp.y, im);
-        }
-        const float r2 = wr * sc[fi], i2 = wi * sc[fi];
-        accT[v][fi].x = fmaf(re, r2, fmaf(-im, i2, accT[v][fi].x));
-        accT[v][fi].y = fmaf(re, i2, fmaf(im, r2, accT[v][fi].y));
-      }
-    }
-  }
-}
+// Transmit terms of one (voxel, tilt), cached in LDS for a chunk of tilts.
+struct alignas(16) TxTerm {
+  double bu;   // transmit sample offset fs*((|P-VS| - d0)/c + t0)
+  float2 et;   // e^{+j 2 pi frac(Bp)}, Bp = fd*((|P-VS| - d0 - 2z)/c + t0)
+};
 
-template <int KIND, int V, int FB, typename S>
+// Forward DAS of one workgroup: 256 threads x V voxels, frames
+// [by*FB, by*FB + FB).  Loop order: chunks of a.tc (<= TC) tilts, then
+// receive lines, then the chunk's tilts.  Per (voxel, tilt) the transmit
+// sample offset and phasor are computed once per chunk and cached in LDS;
+// per (voxel, line) the receive distance, Hann weight and receive phasor are
+// computed once and serve every tilt of the chunk.  The phase is the sum of
+// the two range-reduced parts, e^{j2pi(p_tx + p_rx)} = e^{j2pi p_tx} e^{j2pi p_rx}.
+//
+// LDS: the transmit cache [TC][V][256] TxTerm, then per line rows
+// [tilt j][frame fi] of wcap samples: the window of trace (f0 + fi, t0 + j, c)
+// that the workgroup's voxels can reach, bounded from their bounding box and
+// the chunk's tilts (one window for all rows).  A window longer than wcap is
+// staged in chunks (passes); pass k takes the voxels whose first tap lies in
+// [s0 + k*step, s0 + (k+1)*step) (the last pass: from s0 + k*step on).
+//
+// CMP: compound over tilts (acc per voxel and frame).  Otherwise TC = 1 and
+// accT holds the current tilt's sums over the lines, or one line's terms with
+// nosum.
+template <int KIND, int V, int FB, int TC, bool CMP, typename S>
 __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned by,
                                           unsigned char* smem_raw) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
   constexpr int E = 16 / sizeof(S);   // samples per 16-byte chunk
-  S* tr = reinterpret_cast<S*>(smem_raw);   // [FB][wcap]
+  constexpr int R = TC * FB;          // LDS rows at most
+  constexpr int G = kThreads / R;     // threads per row when staging
+  static_assert(R <= 64 && kThreads % R == 0, "rows must divide the workgroup");
+  static_assert(CMP || TC == 1, "per-tilt outputs take one tilt per chunk");
+  TxTerm* txc = reinterpret_cast<TxTerm*>(smem_raw);
   const S* iq = reinterpret_cast<const S*>(a.iq);
   __shared__ float s_box[4][6];
-  bool ok;   // scratch for voxel_of at the stores
+  __shared__ float s_scl[R];          // FP16 trace scales of the staged rows
+  __shared__ double s_uab[TC];        // TX_CHANNEL law: per-tilt delay (samples)
+  __shared__ float2 s_eab[TC];        // and phasor
 
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
   const int f0 = by * FB;
   const int nf = min(FB, a.n_frame - f0);
   const int ns = a.n_sample;
   const int wcap = a.wcap;
-  // chunk advance; at least one 16-byte chunk so the chunk loop always ends
+  const int tcn = CMP ? a.tc : 1;   // tilts per chunk (<= TC)
+  S* tr = reinterpret_cast<S*>(smem_raw + (size_t)tcn * V * kThreads * sizeof(TxTerm));
+  // pass advance; at least one 16-byte chunk so the pass loop always ends
   // (rows shorter than NT + E only occur with traces too short for any tap)
   const int step = max((wcap - NT + 1) / E * E, E);
-  const long long fstride = (long long)a.n_tx * a.n_ch * ns;
+  const long long tstride = (long long)a.n_ch * ns;
+  const bool rx = a.receive_x != 0;
+  const bool ab_tc = a.ab_mode == RCA_ABLAW_TX_CHANNEL;
+  bool ok;   // scratch for voxel_of at the stores
 
-  double px[V], py[V], pz[V], apF[V], iapF[V];
+  // receive-axis and transmit-axis coordinates, depth, aperture half-width
+  double pl[V], pt[V], pz[V], apF[V], iapF[V];
   bool valid[V];
-  float2 acc[V][FB];
-  float macc[V];
   float bb[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
 #pragma unroll
   for (int v = 0; v < V; ++v) {
     const long long p = voxel_of(a, bx, V, v, tid, valid[v]);
     const long long pc = valid[v] ? p : 0;
     const float fx = a.gx[pc], fy = a.gy[pc], fz = a.gz[pc];
-    px[v] = fx;
-    py[v] = fy;
+    pl[v] = rx ? fx : fy;
+    pt[v] = rx ? fy : fx;
     pz[v] = fz;
     apF[v] = pz[v] / (double)a.fnum;   // .cu:174
     iapF[v] = (double)a.fnum / pz[v];
-    macc[v] = 0.0f;
-#pragma unroll
-    for (int fi = 0; fi < FB; ++fi) acc[v][fi] = make_float2(0.0f, 0.0f);
     if (valid[v]) {
       bb[0] = fminf(bb[0], fx); bb[1] = fmaxf(bb[1], fx);
       bb[2] = fminf(bb[2], fy); bb[3] = fmaxf(bb[3], fy);
@@ -513,7 +487,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
       bb[k] = (k & 1) ? fmaxf(bb[k], y) : fminf(bb[k], y);
     }
   }
-  if ((tid & 63) == 0) {
+  if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) s_box[tid >> 6][k] = bb[k];
   }
@@ -525,135 +499,233 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
     for (int w = 1; w < 4; ++w) r = (k & 1) ? fmaxf(r, s_box[w][k]) : fminf(r, s_box[w][k]);
     bb[k] = uni(r);
   }
-  // receive-axis, transmit-axis and depth ranges (held in scalar registers)
-  const double lat_lo = uni((double)(a.receive_x ? bb[0] : bb[2]));
-  const double lat_hi = uni((double)(a.receive_x ? bb[1] : bb[3]));
-  const double tx_lo = uni((double)(a.receive_x ? bb[2] : bb[0]));
-  const double tx_hi = uni((double)(a.receive_x ? bb[3] : bb[1]));
-  const double zlo = uni((double)bb[4]), zhi = uni((double)bb[5]);
-  const double apmax = uni(a.fnum > 0.0f ? zhi / (double)a.fnum : zlo / (double)a.fnum);
+  // receive-axis, transmit-axis and depth ranges
+  const float lat_lo = rx ? bb[0] : bb[2], lat_hi = rx ? bb[1] : bb[3];
+  const float txl_lo = rx ? bb[2] : bb[0], txl_hi = rx ? bb[3] : bb[1];
+  const float zlo = bb[4], zhi = bb[5];
+  const double apmax = uni(a.fnum > 0.0f ? (double)zhi / (double)a.fnum : (double)zlo / (double)a.fnum);
 
-  for (int t = 0; t < a.n_tx; ++t) {
-    const float vxf = a.vx[t], vyf = a.vy[t], vzf = a.vz[t];
-    const double d0 = a.d0[t];
-    double Bu[V], Bp[V];
-    bool act[V];
+  float2 acc[V][FB];
+  float macc[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    macc[v] = 0.0f;
+#pragma unroll
+    for (int fi = 0; fi < FB; ++fi) acc[v][fi] = make_float2(0.0f, 0.0f);
+  }
+
+  for (int t0 = 0; t0 < a.n_tx; t0 += tcn) {
+    const int nt = min(tcn, a.n_tx - t0);
+    // transmit terms of the chunk's tilts, per voxel, into the cache; the
+    // tilts that insonify each voxel (.cu:191-214) as bits
+    unsigned act[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) act[v] = 0u;
+    for (int j = 0; j < nt; ++j) {
+      const int t = t0 + j;
+      const float vxf = a.vx[t], vyf = a.vy[t], vzf = a.vz[t];
+      const double d0 = a.d0[t];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        if (!valid[v]) continue;
+        // .cu:256-269: |P - VS| with the receive-axis offset dropped
+        const double dl = pt[v] - (double)(rx ? vyf : vxf), dz = pz[v] - (double)vzf;
+        const double tt = sqrt_nr(fma(dl, dl, dz * dz)) - d0;
+        const double Bp = fma(a.kfd, fma(-2.0, pz[v], tt), a.pt0);
+        float sn, cs;
+        sincos2pi((float)(Bp - rint(Bp)), sn, cs);
+        TxTerm x;
+        x.bu = fma(a.kfs, tt, a.ut0);
+        x.et = make_float2(cs, sn);
+        txc[(j * V + v) * kThreads + tid] = x;
+        const float fx = (float)(rx ? pl[v] : pt[v]), fy = (float)(rx ? pt[v] : pl[v]);
+        if (!a.mask_tx || tx_mask(a, vxf, vyf, vzf, fx, fy, (float)pz[v])) act[v] |= 1u << j;
+      }
+    }
+    // transmit sample-offset bounds of the chunk's tilts over the box (lane
+    // L < nt takes tilt t0 + L, then a wave reduction)
+    float bu_lo = INFINITY, bu_hi = -INFINITY;
+    if (lane < nt) {
+      const int t = t0 + lane;
+      tx_bounds(a, txl_lo, txl_hi, zlo, zhi, rx ? a.vy[t] : a.vx[t], a.vz[t], a.d0[t], bu_lo, bu_hi);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      bu_lo = fminf(bu_lo, __shfl_xor(bu_lo, o, 64));
+      bu_hi = fmaxf(bu_hi, __shfl_xor(bu_hi, o, 64));
+    }
+    bu_lo = uni(bu_lo);
+    bu_hi = uni(bu_hi);
+
     float2 accT[V][FB];
     float mT[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-      tx_terms(a, px[v], py[v], pz[v], vxf, vyf, vzf, d0, Bu[v], Bp[v]);
-      act[v] = valid[v] && (!a.mask_tx || tx_mask(a, vxf, vyf, vzf, (float)px[v], (float)py[v],
-                                                   (float)pz[v]));
       mT[v] = 0.0f;
 #pragma unroll
       for (int fi = 0; fi < FB; ++fi) accT[v][fi] = make_float2(0.0f, 0.0f);
     }
-    // transmit sample offset over the box: Bu is increasing in the distance
-    double BuMin, BuMax;
-    {
-      double l0, l1, z0, z1;
-      abs_range(tx_lo - (double)(a.receive_x ? vyf : vxf), tx_hi - (double)(a.receive_x ? vyf : vxf), l0, l1);
-      abs_range(zlo - (double)vzf, zhi - (double)vzf, z0, z1);
-      BuMin = uni(fma(a.kfs, sqrt(l0 * l0 + z0 * z0) - d0, a.ut0));
-      BuMax = uni(fma(a.kfs, sqrt(l1 * l1 + z1 * z1) - d0, a.ut0));
-    }
 
     for (int c = 0; c < a.n_ch; ++c) {
       const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
-      double uab, pab;
-      ab_terms(a, t, c, uab, pab);
-      // sample window of this (tilt, line) over the box; lines whose aperture
-      // misses every voxel (min |lateral| >= max z/F) contribute nothing
+      const double e_lat = rx ? exd : eyd;
+      double uabc = 0.0, pabc = 0.0;
+      if (a.ab_mode == RCA_ABLAW_CHANNEL) {
+        const double ab = a.ab[c];
+        uabc = a.fs * ab;
+        pabc = a.fd * ab;
+      }
+      // receive bounds of line c over the box; a line whose aperture misses
+      // every voxel (min |lateral| >= max z/F) contributes nothing
       double A0, A1, Z0, Z1;
-      abs_range(lat_lo - (a.receive_x ? exd : eyd), lat_hi - (a.receive_x ? exd : eyd), A0, A1);
-      abs_range(zlo - ezd, zhi - ezd, Z0, Z1);
-      const bool any_ap = uni((int)(A0 < apmax)) != 0;
-      int s0 = 0, w_hi = 0;
-      if (any_ap) {
-        A1 = fmin(A1, apmax);
-        const double umin = BuMin + a.kfs * sqrt(A0 * A0 + Z0 * Z0) + uab;
-        const double umax = BuMax + a.kfs * sqrt(A1 * A1 + Z1 * Z1) + uab;
-        // taps [floor(u) - LO, floor(u) + HI]; one sample of margin on each
-        // side covers rounding in the bounds
-        const int w_lo = (int)fmin(fmax(0.0, floor(umin) - (LO + 1)), (double)ns);
-        w_hi = (int)fmax(fmin((double)ns, floor(umax) + (HI + 2)), 0.0);
-        s0 = (w_lo / E) * E;
-      }
-      s0 = uni(s0);
-      w_hi = uni(w_hi);
-      const S* gtr = iq + ((long long)f0 * a.n_tx * a.n_ch + (long long)t * a.n_ch + c) * ns;
-      float sc[FB];
-#pragma unroll
-      for (int fi = 0; fi < FB; ++fi) {
-        sc[fi] = 1.0f;
-        if constexpr (sizeof(S) == 4) {
-          if (fi < nf) sc[fi] = a.scale[((long long)(f0 + fi) * a.n_tx + t) * a.n_ch + c];
-        }
-      }
-      if (a.nosum) {   // per-line output: the partial sum restarts at every line
+      abs_range((double)lat_lo - e_lat, (double)lat_hi - e_lat, A0, A1);
+      abs_range((double)zlo - ezd, (double)zhi - ezd, Z0, Z1);
+      const bool any_ap = A0 < apmax;
+      if (!CMP && a.nosum) {   // per-line outputs: the sums restart at every line
 #pragma unroll
         for (int v = 0; v < V; ++v)
 #pragma unroll
           for (int fi = 0; fi < FB; ++fi) accT[v][fi] = make_float2(0.0f, 0.0f);
       }
-      // The window [s0, w_hi) is staged in chunks of at most wcap samples: one
-      // chunk unless the window is longer than the LDS rows.  Chunk [cb, ce)
-      // takes the voxels whose first tap lies in [cb, cb + step) (the last
-      // chunk: from cb on), and it holds all of their taps.
-      int cb = s0;
-      if (any_ap) do {
-        const int ce = min(cb + wcap, w_hi);
-        const bool first = cb == s0, last = ce == w_hi;
-        __syncthreads();
-        stage_window<S>(tr, gtr, fstride, nf, cb, ce - cb, wcap, a.vec_ok);
-        __syncthreads();
-        const int b_hi = last ? ns : cb + step;
+      if (any_ap) {
+        A1 = fmin(A1, apmax);
+        float r_lo = (float)(a.kfs * sqrt(A0 * A0 + Z0 * Z0) + uabc);
+        float r_hi = (float)(a.kfs * sqrt(A1 * A1 + Z1 * Z1) + uabc);
+        if (ab_tc) {   // per-tilt delays of the TX_CHANNEL law: their range over the chunk
+          float lo = INFINITY, hi = -INFINITY;
+          if (lane < nt) lo = hi = (float)(a.fs * a.ab[(long long)(t0 + lane) * a.n_ch + c]);
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-          if (!act[v]) continue;
-          const Upd g = update_geom<LO, HI>(a, px[v], py[v], pz[v], apF[v], iapF[v], Bu[v], Bp[v],
-                                            exd, eyd, ezd, uab, pab);
-          if (first) mT[v] += g.w;   // .cu:301-304 (inside the aperture, before the range test)
-          if (!g.in_rng) continue;
-          const int base = g.m - LO;
-          if ((!first && base < cb) || base >= b_hi) continue;
-          float q[NT];
-          interp_weights<KIND>(g.d, q);
-          float sn, cs;
-          sincos2pi(g.p, sn, cs);
-          const float wr = g.w * cs, wi = g.w * sn;
-          // the bounds put every tap inside the chunk; the clamp only keeps
-          // LDS reads inside the rows
-          const S* trv = tr + min(max(base - cb, 0), wcap - NT);
-#pragma unroll
-          for (int fi = 0; fi < FB; ++fi) {
-            if (fi < nf) {
-              const S* trf = trv + fi * wcap;
-              float re = 0.0f, im = 0.0f;
-#pragma unroll
-              for (int k = 0; k < NT; ++k) {
-                const float2 smp = ld_sample(trf, k);
-                re = fmaf(q[k], smp.x, re);
-                im = fmaf(q[k], smp.y, im);
+          for (int o = 32; o > 0; o >>= 1) {
+            lo = fminf(lo, __shfl_xor(lo, o, 64));
+            hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+          }
+          r_lo += uni(lo);
+          r_hi += uni(hi);
+        }
+        // one window [s0, whi) for all the chunk's tilts: taps floor(u) - LO ..
+        // floor(u) + HI, with one sample of margin on each side for rounding
+        const int w_lo = (int)fminf(fmaxf(0.0f, floorf(bu_lo + r_lo) - (float)(LO + 1)), (float)ns);
+        const int whi = uni((int)fmaxf(fminf((float)ns, floorf(bu_hi + r_hi) + (float)(HI + 2)), 0.0f));
+        const int s0 = uni((w_lo / E) * E);
+        const int npass = whi - s0 > wcap ? 1 + (whi - s0 - wcap + step - 1) / step : 1;
+
+        for (int k = 0; k < npass; ++k) {
+          const int cb = s0 + k * step;
+          const bool last = cb + wcap >= whi;
+          // stage rows [j][fi] (G threads per row), with their FP16 scales and
+          // the TX_CHANNEL law's per-tilt delays and phasors
+          __syncthreads();
+          {
+            const int row = tid / G, li = tid % G;
+            const int j = row / FB, fi = row % FB;
+            const int len = min(cb + wcap, whi) - cb;
+            if (j < nt && fi < nf && len > 0) {
+              const S* src = iq + ((long long)(f0 + fi) * a.n_tx + t0 + j) * tstride +
+                             (long long)c * ns + cb;
+              S* dst = tr + row * wcap;
+              if (a.vec_ok) {
+                const int n16 = (int)(((long long)len * sizeof(S) + 15) / 16);
+                const uint4* s16 = reinterpret_cast<const uint4*>(src);
+                uint4* d16 = reinterpret_cast<uint4*>(dst);
+                for (int i = li; i < n16; i += G) d16[i] = s16[i];
+              } else {
+                for (int i = li; i < len; i += G) dst[i] = src[i];
               }
-              const float r2 = wr * sc[fi], i2 = wi * sc[fi];
-              accT[v][fi].x = fmaf(re, r2, fmaf(-im, i2, accT[v][fi].x));
-              accT[v][fi].y = fmaf(re, i2, fmaf(im, r2, accT[v][fi].y));
+            }
+            if constexpr (sizeof(S) == 4) {
+              if (li == 0 && j < nt && fi < nf)
+                s_scl[row] = a.scale[((long long)(f0 + fi) * a.n_tx + t0 + j) * a.n_ch + c];
+            }
+            if (ab_tc && k == 0 && tid < nt) {
+              const double ab = a.ab[(long long)(t0 + tid) * a.n_ch + c];
+              const double pab = a.fd * ab;
+              float sn, cs;
+              sincos2pi((float)(pab - rint(pab)), sn, cs);
+              s_uab[tid] = a.fs * ab;
+              s_eab[tid] = make_float2(cs, sn);
+            }
+          }
+          __syncthreads();
+
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            if (!act[v]) continue;
+            const double A = fabs(pl[v] - e_lat);   // .cu:284-291
+            if (!(A < apF[v])) continue;            // .cu:294
+            const float w = hann((float)(A * iapF[v]));   // .cu:299
+            // .cu:301-304: every insonifying tilt, inside the aperture, before
+            // the range test
+            if (k == 0) {
+              if constexpr (CMP) macc[v] = fmaf(w, (float)__popc(act[v]), macc[v]);
+              else mT[v] += w;
+            }
+            const double dz = pz[v] - ezd;
+            const double Rr = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));   // .cu:308-309
+            const double ur = fma(a.kfs, Rr, uabc);                         // .cu:310-311
+            const double ph = fma(a.kfd, Rr, pabc);                         // .cu:371-373
+            float sn, cs;
+            sincos2pi((float)(ph - rint(ph)), sn, cs);
+            const float er = w * cs, ei = w * sn;
+            const TxTerm* tv = txc + v * kThreads + tid;
+            const S* rows = tr;
+#pragma unroll 2
+            for (int j = 0; j < nt; ++j, tv += V * kThreads, rows += FB * wcap) {
+              if (!((act[v] >> j) & 1u)) continue;
+              const TxTerm x = *tv;
+              double u = x.bu + ur;
+              if (ab_tc) u += s_uab[j];
+              const double mf = floor(u);                                       // .cu:319
+              if (!(mf >= (double)LO && mf < (double)(ns - 1 - HI))) continue;  // .cu:323
+              const int base = (int)mf - LO;
+              if ((k > 0 && base < cb) || (!last && base >= cb + step)) continue;
+              float q[NT];
+              interp_weights<KIND>((float)(u - mf), q);
+              float pr = er * x.et.x - ei * x.et.y;
+              float pi = er * x.et.y + ei * x.et.x;
+              if (ab_tc) {
+                const float2 eab = s_eab[j];
+                const float r2 = pr * eab.x - pi * eab.y;
+                pi = pr * eab.y + pi * eab.x;
+                pr = r2;
+              }
+              // the bounds put every tap inside the row; the clamp only keeps
+              // LDS reads inside it
+              const S* trv = rows + min(max(base - cb, 0), wcap - NT);
+#pragma unroll
+              for (int fi = 0; fi < FB; ++fi) {
+                if (fi < nf) {
+                  const S* trf = trv + fi * wcap;
+                  float re = 0.0f, im = 0.0f;
+#pragma unroll
+                  for (int kk = 0; kk < NT; ++kk) {
+                    const float2 smp = ld_sample(trf, kk);
+                    re = fmaf(q[kk], smp.x, re);
+                    im = fmaf(q[kk], smp.y, im);
+                  }
+                  float r2 = pr, i2 = pi;
+                  if constexpr (sizeof(S) == 4) {
+                    const float sc = s_scl[j * FB + fi];
+                    r2 *= sc;
+                    i2 *= sc;
+                  }
+                  float2& dst = CMP ? acc[v][fi] : accT[v][fi];
+                  dst.x = fmaf(re, r2, fmaf(-im, i2, dst.x));
+                  dst.y = fmaf(re, i2, fmaf(im, r2, dst.y));
+                }
+              }
             }
           }
         }
-        cb += step;
-      } while (cb < w_hi);
-      if (a.nosum) {   // NoSum.cu:228-229,275-278,376-380: out[c][f][t][p]
+      }
+      if (!CMP && a.nosum) {   // NoSum.cu:228-229,275-278,376-380: out[c][f][t][p]
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           if (!valid[v]) continue;
+          const long long p = voxel_of(a, bx, V, v, tid, ok);
 #pragma unroll
           for (int fi = 0; fi < FB; ++fi) {
             if (fi < nf) {
-              float2* o = a.out + (((long long)c * a.n_frame + f0 + fi) * a.n_tx + t) * a.n_pix +
-                          voxel_of(a, bx, V, v, tid, ok);
+              float2* o = a.out + (((long long)c * a.n_frame + f0 + fi) * a.n_tx + t0) * a.n_pix + p;
               float2 r = accT[v][fi];
               if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
               *o = r;
@@ -663,53 +735,46 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
       }
     }
 
-    if (a.compound) {
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        macc[v] += mT[v];
-#pragma unroll
-        for (int fi = 0; fi < FB; ++fi) {
-          acc[v][fi].x += accT[v][fi].x;
-          acc[v][fi].y += accT[v][fi].y;
-        }
-      }
-    } else {
+    if (!CMP) {   // per-tilt output and mask of tilt t0
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         if (!valid[v]) continue;
+        const long long p = voxel_of(a, bx, V, v, tid, ok);
+        if (!a.nosum) {
 #pragma unroll
-        for (int fi = 0; fi < FB; ++fi) {
-          if (fi < nf && !a.nosum) {
-            float2* o = a.out + ((long long)(f0 + fi) * a.n_tx + t) * a.n_pix +
-                        voxel_of(a, bx, V, v, tid, ok);
-            float2 r = accT[v][fi];
-            if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
-            *o = r;
+          for (int fi = 0; fi < FB; ++fi) {
+            if (fi < nf) {
+              float2* o = a.out + ((long long)(f0 + fi) * a.n_tx + t0) * a.n_pix + p;
+              float2 r = accT[v][fi];
+              if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
+              *o = r;
+            }
           }
         }
         if (a.mask && by == 0) {
-          float* mk = a.mask + (long long)t * a.n_pix + voxel_of(a, bx, V, v, tid, ok);
+          float* mk = a.mask + (long long)t0 * a.n_pix + p;
           *mk = a.accumulate ? *mk + mT[v] : mT[v];
         }
       }
     }
   }
 
-  if (a.compound) {
+  if (CMP) {
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       if (!valid[v]) continue;
+      const long long p = voxel_of(a, bx, V, v, tid, ok);
 #pragma unroll
       for (int fi = 0; fi < FB; ++fi) {
         if (fi < nf) {
-          float2* o = a.out + (long long)(f0 + fi) * a.n_pix + voxel_of(a, bx, V, v, tid, ok);
+          float2* o = a.out + (long long)(f0 + fi) * a.n_pix + p;
           float2 r = acc[v][fi];
           if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
           *o = r;
         }
       }
       if (a.mask && by == 0) {
-        float* mk = a.mask + voxel_of(a, bx, V, v, tid, ok);
+        float* mk = a.mask + p;
         *mk = a.accumulate ? *mk + macc[v] : macc[v];
       }
     }
@@ -717,10 +782,19 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   __syncthreads();   // s_box and the LDS rows may be reused by the caller's next block
 }
 
-template <int KIND, int V, int FB, typename S>
+// Most tilts per chunk: rows of TC x FB windows fit 32 KB at 256 samples.
+#ifndef RCA_TC_MAX
+#define RCA_TC_MAX 8
+#endif
+__host__ __device__ constexpr int tc_max(int fb, int sbytes) {
+  const int r = (sbytes == 8 ? 16 : 32) / fb;
+  return r > RCA_TC_MAX ? RCA_TC_MAX : (r < 1 ? 1 : r);
+}
+
+template <int KIND, int V, int FB, bool CMP, typename S>
 __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  fwd_block<KIND, V, FB, S>(a, blockIdx.x, blockIdx.y, smem_raw);
+  fwd_block<KIND, V, FB, CMP ? tc_max(FB, sizeof(S)) : 1, CMP, S>(a, blockIdx.x, blockIdx.y, smem_raw);
 }
 
 // ----------------------------------------------------------------------------
@@ -731,7 +805,8 @@ __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
 //   items   { first arg, number of halves (1|2), voxel block, frame block }
 // Workgroup g takes items g, g + gridDim.x, ...  Both halves of an item run in
 // the same workgroup, the second accumulating onto the first, so items never
-// race on an output.  FP32 items use FB = 1 and FP16 items FB = 2.
+// race on an output.  FP32 items use V = kBatchV32, FB = 1; FP16 items
+// V = 1, FB = kBatchFB16.
 // ----------------------------------------------------------------------------
 struct BatchArgs {
   KArgs k;
@@ -739,7 +814,8 @@ struct BatchArgs {
   int pad_[3];
 };
 
-constexpr int kBatchV = 2;
+constexpr int kBatchV32 = 1;
+constexpr int kBatchFB16 = 2;
 
 template <int KIND>
 __global__ void __launch_bounds__(kThreads) das_fwd_batch(const int* __restrict__ header,
@@ -752,9 +828,9 @@ __global__ void __launch_bounds__(kThreads) das_fwd_batch(const int* __restrict_
     for (int h = 0; h < it.y; ++h) {
       const KArgs k = args[it.x + h].k;
       if (args[it.x + h].iq_dtype == RCA_IQ_C64)
-        fwd_block<KIND, kBatchV, 1, float2>(k, it.z, it.w, smem_raw);
+        fwd_block<KIND, kBatchV32, 1, tc_max(1, 8), true, float2>(k, it.z, it.w, smem_raw);
       else
-        fwd_block<KIND, kBatchV, 2, __half2>(k, it.z, it.w, smem_raw);
+        fwd_block<KIND, 1, kBatchFB16, tc_max(kBatchFB16, 4), true, __half2>(k, it.z, it.w, smem_raw);
       __syncthreads();
     }
   }
@@ -843,7 +919,7 @@ __global__ void fill_zero(float* __restrict__ p, long long n) {
 // ----------------------------------------------------------------------------
 // Host side.
 // ----------------------------------------------------------------------------
-int g_tune_v = 0, g_tune_fb = 0, g_tune_split = 0;
+int g_tune_v = 0, g_tune_fb = 0, g_tune_split = 0, g_tune_lds = 0;
 
 int check_desc(const rca_das_desc* d) {
   if (!d) return fail(RCA_ERR_INVALID, "desc is NULL");
@@ -885,41 +961,63 @@ KArgs make_args(const rca_das_desc* d, int n_frame) {
   return a;
 }
 
-// LDS samples per frame row: the whole trace (rounded up to 16 bytes) unless
-// workgroups are depth tiles, whose windows are short; then rows are capped
-// so FB rows take at most kWinLds bytes.
-constexpr int kWinLds = 32 * 1024;
-int pick_wcap(const KArgs& a, int fb, size_t sbytes) {
+// LDS layout of a forward launch: tilts per chunk (tc) and samples per row
+// (wcap).  Per tilt of a chunk the workgroup holds V x 256 transmit terms and
+// FB rows.  Depth tiles stage short windows (rows of kTileRow samples; longer
+// windows take several passes); otherwise rows hold whole traces (rounded up
+// to 16 bytes).  tc is what fits the budget, at least 1.
+constexpr int kTileRow = 256;
+struct FwdLayout { int tc, wcap; size_t lds; };
+FwdLayout fwd_layout(const KArgs& a, int V, int fb, size_t sbytes, int tc_kernel, size_t budget) {
   const int e = (int)(16 / sbytes);
   const int full = (a.n_sample + e - 1) / e * e;
-  if (!a.depth_run) return full;
-  const int cap = (int)(kWinLds / ((size_t)fb * sbytes)) / e * e;
-  return cap < full ? cap : full;
+  FwdLayout L;
+  L.wcap = a.depth_run && full > kTileRow ? kTileRow : full;
+  const size_t per_tilt = (size_t)V * kThreads * sizeof(TxTerm) + (size_t)fb * L.wcap * sbytes;
+  const long long fit = (long long)(budget / per_tilt);
+  L.tc = (int)(fit < 1 ? 1 : fit > tc_kernel ? tc_kernel : fit);
+  L.lds = (size_t)L.tc * per_tilt;
+  return L;
 }
+
+// LDS budget of a forward workgroup (measured on cfg1-sized volumes):
+// single-frame blocks run best at 40 KB (4 workgroups per CU, 6 tilts per
+// chunk), multi-frame blocks at 48 KB.
+constexpr size_t kFwdLds = 48 * 1024, kFwdLds1 = 40 * 1024;
 
 template <typename S> struct SName;
 
-template <int KIND, int V, int FB, typename S>
+template <int KIND, int V, int FB, bool CMP, typename S>
 int launch_fwd_t(KArgs a, hipStream_t st) {
-  a.wcap = pick_wcap(a, FB, sizeof(S));
-  const size_t lds = (size_t)FB * a.wcap * sizeof(S);
-  if (lds > (size_t)kMaxLds) return fail(RCA_ERR_UNSUPPORTED, "FB=%d traces do not fit in LDS", FB);
+  const FwdLayout L = fwd_layout(a, V, FB, sizeof(S), CMP ? tc_max(FB, sizeof(S)) : 1,
+                                 g_tune_lds ? (size_t)g_tune_lds * 1024 : FB == 1 ? kFwdLds1 : kFwdLds);
+  if (L.lds > (size_t)kMaxLds)
+    return fail(RCA_ERR_UNSUPPORTED, "FB=%d traces of %d samples do not fit in LDS", FB, a.n_sample);
+  a.tc = L.tc;
+  a.wcap = L.wcap;
   const unsigned gx = (unsigned)fwd_blocks(a.n_pix, a.depth_run, V);
   const unsigned gy = (unsigned)((a.n_frame + FB - 1) / FB);
   if (gx == 0 || gy == 0) return RCA_OK;
-  hipLaunchKernelGGL((das_fwd<KIND, V, FB, S>), dim3(gx, gy, 1), dim3(kThreads), lds, st, a);
+  hipLaunchKernelGGL((das_fwd<KIND, V, FB, CMP, S>), dim3(gx, gy, 1), dim3(kThreads), L.lds, st, a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "das_fwd launch");
 }
 
+// Instantiated shapes: compounding (V, FB) in (1|2, 1), (1, 2|4|8); per-tilt
+// and per-line outputs (1, 1|4).
 template <int KIND, typename S>
 int launch_fwd_k(const KArgs& a, int V, int FB, hipStream_t st) {
+  if (a.compound) {
 #define RCA_FWD_CASE(VV, FF) \
-  if (V == VV && FB == FF) return launch_fwd_t<KIND, VV, FF, S>(a, st);
-  RCA_FWD_CASE(1, 1) RCA_FWD_CASE(2, 1) RCA_FWD_CASE(4, 1)
-  RCA_FWD_CASE(1, 2) RCA_FWD_CASE(2, 2) RCA_FWD_CASE(1, 4) RCA_FWD_CASE(2, 4)
-  RCA_FWD_CASE(1, 8)
+    if (V == VV && FB == FF) return launch_fwd_t<KIND, VV, FF, true, S>(a, st);
+    RCA_FWD_CASE(1, 1) RCA_FWD_CASE(2, 1) RCA_FWD_CASE(1, 2) RCA_FWD_CASE(1, 4) RCA_FWD_CASE(1, 8)
 #undef RCA_FWD_CASE
+  } else {
+#define RCA_FWD_CASE(FF) \
+    if (V == 1 && FB == FF) return launch_fwd_t<KIND, 1, FF, false, S>(a, st);
+    RCA_FWD_CASE(1) RCA_FWD_CASE(4)
+#undef RCA_FWD_CASE
+  }
   return fail(RCA_ERR_UNSUPPORTED, "no kernel for V=%d FB=%d", V, FB);
 }
 
@@ -935,17 +1033,17 @@ int launch_fwd_s(const KArgs& a, int interp, int V, int FB, hipStream_t st) {
 }
 
 int pick_fb(int n_frame, int n_sample, size_t bytes, bool tiled) {
-  int fb = 8;
+  int fb = bytes == 8 ? 4 : 8;   // measured best: FP32 4 frames, FP16 8 frames
   // depth tiles stage short windows, so any FB fits; whole traces must fit
-  while (fb > 1 && (fb > n_frame || (!tiled && (size_t)fb * n_sample * bytes > (size_t)kMaxLds))) fb >>= 1;
+  while (fb > 1 && (fb > n_frame || (!tiled && (size_t)fb * n_sample * bytes +
+                                      kThreads * sizeof(TxTerm) > (size_t)kMaxLds))) fb >>= 1;
   return fb;
 }
 
-int pick_v(long long n_pix, int depth_run, int n_fblocks, int fb) {
-  // V = 2 keeps the kernel at 3 waves per SIMD (measured fastest); V = 1 when
-  // that would leave fewer than ~4 workgroups per CU (256 CUs on MI355X).
-  const long long blocks2 = fwd_blocks(n_pix, depth_run, 2) * n_fblocks;
-  return (fb <= 4 && blocks2 >= 1024) ? 2 : 1;
+int pick_v(long long, int, int, int, bool) {
+  // V = 1: the tilt-chunked kernel holds more per-voxel state, and one voxel
+  // per lane measured fastest at every LDS budget on cfg1-sized volumes.
+  return 1;
 }
 
 template <int KIND, int V, int FB>
@@ -978,13 +1076,14 @@ const char* rca_version(void) { return "rca_das 0.1.0 gfx950"; }
 
 // Tuning override for sweeps: workgroup voxels per lane (V), frames per block
 // (FB) and adjoint voxel splits; 0 restores the automatic choice.
-int rca_set_tuning(int v, int fb, int split) {
+int rca_set_tuning(int v, int fb, int split, int lds_kib) {
   if ((v != 0 && v != 1 && v != 2 && v != 4) || (fb != 0 && fb != 1 && fb != 2 && fb != 4 && fb != 8) ||
-      split < 0)
-    return fail(RCA_ERR_INVALID, "bad tuning V=%d FB=%d split=%d", v, fb, split);
+      split < 0 || (lds_kib != 0 && (lds_kib < 16 || lds_kib > 64)))
+    return fail(RCA_ERR_INVALID, "bad tuning V=%d FB=%d split=%d lds=%d KiB", v, fb, split, lds_kib);
   g_tune_v = v;
   g_tune_fb = fb;
   g_tune_split = split;
+  g_tune_lds = lds_kib;
   return RCA_OK;
 }
 
@@ -1005,7 +1104,13 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
     return fail(RCA_ERR_INVALID, "RCA_FLAG_NOSUM excludes RCA_FLAG_COMPOUND");
   if (n_frame == 0 || d->n_pix == 0) return RCA_OK;
   const size_t sbytes = iq_dtype == RCA_IQ_C64 ? 8 : 4;
-  if ((size_t)d->n_sample * sbytes > (size_t)kMaxLds)
+  const bool tiled = d->depth_run >= 64;
+  // whole-trace rows (no depth tiles) must fit LDS beside one tilt's transmit terms
+  const auto trace_fits = [&](int fb, int v) {
+    return tiled || (size_t)fb * d->n_sample * sbytes + (size_t)v * kThreads * sizeof(TxTerm) <=
+                        (size_t)kMaxLds;
+  };
+  if (!trace_fits(1, 1))
     return fail(RCA_ERR_UNSUPPORTED, "nSample=%d exceeds the LDS trace capacity (%d bytes)",
                 d->n_sample, kMaxLds);
   if (d->ab_law_mode == RCA_ABLAW_TX_CHANNEL && !d->ab_law_rx)
@@ -1021,8 +1126,12 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
   a.vec_ok = (((uintptr_t)iq & 15) == 0) && (((size_t)d->n_sample * sbytes) % 16 == 0);
   int fb = g_tune_fb ? g_tune_fb : pick_fb(n_frame, d->n_sample, sbytes, a.depth_run != 0);
   const int nfb = (n_frame + fb - 1) / fb;
-  int v = g_tune_v ? g_tune_v : pick_v(d->n_pix, a.depth_run, nfb, fb);
-  if ((v == 4 && fb > 1) || (v == 2 && fb > 4)) v = 1;   // instantiated shapes
+  int v = g_tune_v ? g_tune_v : pick_v(d->n_pix, a.depth_run, nfb, fb, a.compound != 0);
+  if (v > 2) v = 2;                                  // instantiated shapes
+  if (!a.compound || fb > 1) v = 1;
+  if (!a.compound && fb > 1) fb = fb >= 4 ? 4 : 1;
+  while (fb > 1 && !trace_fits(fb, v)) fb >>= 1;    // a tuned FB that does not fit
+  if (!trace_fits(fb, v)) v = 1;
   if (iq_dtype == RCA_IQ_C64) return launch_fwd_s<float2>(a, d->interp, v, fb, stream);
   return launch_fwd_s<__half2>(a, d->interp, v, fb, stream);
 }
@@ -1045,7 +1154,8 @@ int rca_adjoint(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_
   a.gin = reinterpret_cast<float2*>(grad_iq);
   a.gout_tx = grad_tx;
   a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
-  const int fb = pick_fb(n_frame, d->n_sample, 8, false);
+  int fb = 8;   // adjoint: FB x nSample complex FP32 accumulators in LDS
+  while (fb > 1 && (fb > n_frame || (size_t)fb * d->n_sample * 8 > (size_t)kMaxLds)) fb >>= 1;
   const int nfb = (n_frame + fb - 1) / fb;
   const long long traces = (long long)d->n_channel * d->n_tx * nfb;
   int split = g_tune_split;
@@ -1111,6 +1221,17 @@ int bfVirtualSourceGeneric_RCA(float* dataBf, float* mask, const float* data,
 // ---------------------------------------------------------------------------
 // Batched launches for the serving loop (persistent, graph-capturable).
 // ---------------------------------------------------------------------------
+// LDS of a batched launch: the forward budget, or one tilt of whole-trace
+// rows when that is larger (FP32 V = kBatchV32, FB = 1; FP16 V = 1,
+// FB = kBatchFB16; 16-byte rows).
+static size_t batch_lds(int n_sample_max) {
+  const size_t ns = ((size_t)n_sample_max + 3) / 4 * 4;
+  const size_t f32 = (size_t)kBatchV32 * kThreads * sizeof(TxTerm) + ns * 8;
+  const size_t f16 = (size_t)kThreads * sizeof(TxTerm) + (size_t)kBatchFB16 * ns * 4;
+  size_t need = f32 > f16 ? f32 : f16;
+  return need > kFwdLds ? need : kFwdLds;
+}
+
 static size_t batch_args_off() { return 256; }
 static size_t batch_items_off(int max_halves) {
   return (batch_args_off() + (size_t)max_halves * sizeof(BatchArgs) + 255) / 256 * 256;
@@ -1137,7 +1258,7 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
     if (E.n_half < 1 || E.n_half > 2) return fail(RCA_ERR_INVALID, "entry %d: n_half must be 1 or 2", q);
     if (E.iq_dtype != RCA_IQ_C64 && E.iq_dtype != RCA_IQ_F16X2)
       return fail(RCA_ERR_INVALID, "entry %d: unknown iq dtype", q);
-    const int fb = E.iq_dtype == RCA_IQ_C64 ? 1 : 2;
+    const int fb = E.iq_dtype == RCA_IQ_C64 ? 1 : kBatchFB16;
     if (!E.out || E.n_frame < 1 || E.frame_begin < 0 || E.frame_end > E.n_frame ||
         E.frame_begin >= E.frame_end || E.frame_begin % fb ||
         (E.frame_end % fb && E.frame_end != E.n_frame))
@@ -1169,12 +1290,15 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
       a.nosum = 0;
       const size_t sb = E.iq_dtype == RCA_IQ_C64 ? 8 : 4;
       a.vec_ok = (((uintptr_t)E.iq[h] & 15) == 0) && (((size_t)d->n_sample * sb) % 16 == 0);
-      a.wcap = (int)((d->n_sample + 16 / sb - 1) / (16 / sb) * (16 / sb));   // whole trace
+      const FwdLayout L = fwd_layout(a, E.iq_dtype == RCA_IQ_C64 ? kBatchV32 : 1, fb, sb,
+                                     tc_max(fb, (int)sb), batch_lds(n_sample_max));
+      a.tc = L.tc;
+      a.wcap = L.wcap;
       args[na].k = a;
       args[na].iq_dtype = E.iq_dtype;
       ++na;
     }
-    const long long nvb = fwd_blocks(n_pix, depth_run, kBatchV);
+    const long long nvb = fwd_blocks(n_pix, depth_run, E.iq_dtype == RCA_IQ_C64 ? kBatchV32 : 1);
     for (int fbk = E.frame_begin / fb; fbk * fb < E.frame_end; ++fbk) {
       for (long long vb = 0; vb < nvb; ++vb) {
         if (ni >= max_items) return fail(RCA_ERR_UNSUPPORTED, "batch exceeds %d items", max_items);
@@ -1192,9 +1316,7 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_bl
   g_err.clear();
   if (!dev_ws || max_halves < 1 || grid_blocks < 1 || n_sample_max < 1)
     return fail(RCA_ERR_INVALID, "bad batch launch arguments");
-  // rows of whole traces: FP32 FB=1 x 8 B or FP16 FB=2 x 4 B per sample,
-  // each row rounded up to 16 bytes
-  const size_t lds = ((size_t)n_sample_max + 3) / 4 * 4 * 8;
+  const size_t lds = batch_lds(n_sample_max);
   if (lds > (size_t)kMaxLds) return fail(RCA_ERR_UNSUPPORTED, "nSample %d exceeds LDS", n_sample_max);
   const unsigned char* ws = reinterpret_cast<const unsigned char*>(dev_ws);
   const int* hdr = reinterpret_cast<const int*>(ws);

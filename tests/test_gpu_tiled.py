@@ -2,13 +2,16 @@
 
 With the hint, a workgroup owns 64 depths x 4V columns and stages only the
 sample window that its voxels' taps can reach on each (tilt, line).  Its
-results must match the untiled launch (1e-6 relative: the per-voxel arithmetic
-is the same; only the compiler's contraction choices can differ between
-instantiations) and the FP64 oracle (1e-5, 0.1 dB).  Cases: every
+results must match the untiled launch and the FP64 oracle (1e-5, 0.1 dB).
+Tiled and untiled launches add the same per-voxel terms, but the tilts per
+chunk (set by the LDS budget) fix the order in which tilts are added, so
+compounded sums agree to FP32 rounding: TOL = 4e-6 relative L2 (per-tilt
+outputs, whose order does not change, agree to 1e-6).  Cases: every
 interpolation, both halves, per-tilt and compounded output, frame counts that
 leave partial frame blocks, column counts that leave partial column groups, a
 hint that does not divide the voxel count, a depth-coarse grid whose windows
-are longer than the LDS rows (staged in several chunks), FP16-I/Q and NoSum.
+are longer than the LDS rows (staged in several passes), several LDS budgets
+(tilts per chunk), FP16-I/Q and NoSum.
 """
 import numpy as np
 import pytest
@@ -20,6 +23,8 @@ from tests import helpers as H
 from tests.test_gpu_forward import check, oracle_fwd, say
 
 pytestmark = pytest.mark.gpu
+
+TOL = 4e-6
 
 TILE = C.scaled(C.get("cfg1"), nx=5, ny=3, nz=128, n_scat=300)   # 15 columns, 2 depth tiles
 
@@ -50,8 +55,8 @@ def test_tiled_matches_untiled_and_oracle(dev, interp, half_index):
         torch.cuda.synchronize()
         r = H.rel_l2(o_t.cpu().numpy(), o_f.cpu().numpy())
         say(f"tiled vs untiled compound={compound}: rel-L2 {r:.3e}")
-        assert r <= 1e-6
-        assert H.rel_l2(m_t.cpu().numpy(), m_f.cpu().numpy()) <= 1e-6
+        assert r <= (TOL if compound else 1e-6)
+        assert H.rel_l2(m_t.cpu().numpy(), m_f.cpu().numpy()) <= TOL
     ref, rmask = oracle_fwd(TILE, half_index, iq, interp)
     check(o_t.cpu().numpy(), ref.sum(axis=1, keepdims=True), f"tiled {interp}/{half_index} vs oracle")
     assert H.rel_l2(m_t.cpu().numpy(), rmask.sum(axis=0, keepdims=True)) <= 1e-5
@@ -63,12 +68,13 @@ def test_tiled_launch_shapes(dev):
     flat, tiled, G = plans(TILE, 0, dev, "lanczos4")
     base = flat.forward(iq)[0].cpu().numpy()
     try:
-        for v, fb in ((1, 1), (2, 1), (4, 1), (1, 2), (2, 2), (1, 4), (2, 4), (1, 8)):
-            _lib.set_tuning(v, fb, 0)
+        for v, fb, kib in ((1, 1, 0), (2, 1, 0), (4, 1, 0), (1, 2, 0), (2, 2, 0), (1, 4, 0),
+                           (2, 4, 0), (1, 8, 0), (1, 1, 16), (2, 1, 16), (1, 1, 64), (1, 8, 64)):
+            _lib.set_tuning(v, fb, 0, kib)     # kib: LDS budget, i.e. tilts per chunk
             o = tiled.forward(iq)[0].cpu().numpy()
             r = H.rel_l2(o, base)
-            say(f"V={v} FB={fb}: rel-L2 vs untiled {r:.3e}")
-            assert r <= 1e-6, (v, fb)
+            say(f"V={v} FB={fb} LDS={kib} KiB: rel-L2 vs untiled {r:.3e}")
+            assert r <= TOL, (v, fb, kib)
     finally:
         _lib.set_tuning(0, 0, 0)
 
@@ -82,12 +88,12 @@ def test_hint_not_dividing_voxels(dev):
     o_t, m_t = tiled.forward(iq, compound=False)
     torch.cuda.synchronize()
     assert H.rel_l2(o_t.cpu().numpy(), o_f.cpu().numpy()) <= 1e-6
-    assert H.rel_l2(m_t.cpu().numpy(), m_f.cpu().numpy()) <= 1e-6
+    assert H.rel_l2(m_t.cpu().numpy(), m_f.cpu().numpy()) <= TOL
 
 
 def test_windows_longer_than_rows(dev):
     # depths 0.6 mm apart: a 64-depth tile spans 38 mm, about 750 samples per
-    # window, while FB = 8 FP32 rows hold 512 samples (two chunks)
+    # window, while depth-tile rows hold 256 samples (three passes)
     cfg = C.scaled(TILE, nx=4, ny=3)
     x, y, _ = C.grid_axes(cfg)
     z = 5e-3 + np.arange(cfg.nz) * 0.6e-3
@@ -104,7 +110,7 @@ def test_windows_longer_than_rows(dev):
             o = tiled.forward(x8)[0].cpu().numpy()
             r = H.rel_l2(o, base)
             say(f"V={v} FB={fb}: rel-L2 vs untiled {r:.3e}")
-            assert r <= 1e-6, (v, fb)
+            assert r <= TOL, (v, fb)
     finally:
         _lib.set_tuning(0, 0, 0)
     ref, _ = oracle_fwd(cfg, 0, iq, "lanczos4", G=G)
@@ -121,7 +127,7 @@ def test_tiled_fp16_and_nosum(dev):
     torch.cuda.synchronize()
     r = H.rel_l2(o_t.cpu().numpy(), o_f.cpu().numpy())
     say(f"fp16 tiled vs untiled: rel-L2 {r:.3e}")
-    assert r <= 1e-6
+    assert r <= TOL
     n_f = flat.forward(iq[:1], compound=False, nosum=True, want_mask=False)[0]
     n_t = tiled.forward(iq[:1], compound=False, nosum=True, want_mask=False)[0]
     torch.cuda.synchronize()
