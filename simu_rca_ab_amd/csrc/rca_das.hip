@@ -320,29 +320,30 @@ __device__ __forceinline__ void ab_terms(const KArgs& a, int t, int c, double& u
 // ----------------------------------------------------------------------------
 // Forward kernel.
 //
-// One workgroup: 256 lanes x V consecutive voxels x FB frames.  Loop nest
-// tilt -> receive line; for every (tilt, line) the FB traces are staged in
-// LDS once and each lane evaluates its V updates from them.  Per update: FP64
-// receive distance, FP64 sample position and phase (range-reduced before the
-// FP32 conversion), FP32 Hann weight, interpolation weights, 2a LDS taps and a
-// complex FMA into the per-tilt partial sum.  Per-tilt partial sums are added
-// to the compounded result after each tilt (two-level summation).
-//
-// Measured on MI355X this simple nest at V=2 (143 VGPRs, 3 waves/SIMD) beats
-// variants that reuse the receive terms across tilts at the cost of more
-// registers (2 waves/SIMD).
+// One workgroup: 256 lanes x V voxels x FB frames (fwd_block below has the
+// loop nest).  Voxel tiles: with a depth-run hint (the grid's Nz, voxel
+// p = iz + Nz*column) a workgroup owns kTileDepth depths x kTileCols*V
+// columns, so the sample window it stages per (tilt, line) is short; each
+// wave holds 64/kTileCols depths of kTileCols neighbouring columns, so
+// neighbouring lanes read nearby (often equal) samples, which spreads the LDS
+// reads over the banks.  Without the hint a workgroup owns 256V consecutive
+// voxels.
 // ----------------------------------------------------------------------------
-// Voxel of lane (wave w, lane l), slot v in workgroup bx.  With a depth-run
-// hint (the grid's Nz, voxel p = iz + Nz*column) a workgroup owns a tile of
-// 64 depths x 4V columns, so its sample window per (tilt, line) is narrow;
-// otherwise it owns 256V consecutive voxels.
+#ifndef RCA_TILE_COLS
+#define RCA_TILE_COLS 16   // measured best of 4, 8, 16 on cfg1-sized volumes
+#endif
+constexpr int kTileCols = RCA_TILE_COLS;              // columns per wave
+constexpr int kWaveDepth = 64 / kTileCols;            // depths per wave
+constexpr int kTileDepth = 4 * kWaveDepth;            // depths per workgroup
+
 __device__ __forceinline__ long long voxel_of(const KArgs& a, unsigned bx, int V, int v, int tid,
                                               bool& valid) {
   if (a.depth_run >= 64) {
-    const long long ndc = (a.depth_run + 63) / 64;
+    const long long ndc = (a.depth_run + kTileDepth - 1) / kTileDepth;
     const long long dc = bx % ndc, cg = bx / ndc;
-    const long long col = cg * (4 * V) + (tid >> 6) * V + v;
-    const long long dep = dc * 64 + (tid & 63);
+    const int l = tid & 63;
+    const long long col = cg * (kTileCols * V) + v * kTileCols + (l % kTileCols);
+    const long long dep = dc * kTileDepth + (tid >> 6) * kWaveDepth + l / kTileCols;
     const long long p = dep + (long long)a.depth_run * col;
     valid = dep < a.depth_run && p < a.n_pix;
     return p;
@@ -355,7 +356,8 @@ __device__ __forceinline__ long long voxel_of(const KArgs& a, unsigned bx, int V
 __host__ __device__ inline long long fwd_blocks(long long n_pix, int depth_run, int V) {
   if (depth_run >= 64) {
     const long long ncol = (n_pix + depth_run - 1) / depth_run;
-    return ((ncol + 4 * V - 1) / (4 * V)) * ((depth_run + 63) / 64);
+    return ((ncol + kTileCols * V - 1) / (kTileCols * V)) *
+           ((depth_run + kTileDepth - 1) / kTileDepth);
   }
   return (n_pix + kThreads * V - 1) / (kThreads * V);
 }
@@ -966,7 +968,7 @@ KArgs make_args(const rca_das_desc* d, int n_frame) {
 // FB rows.  Depth tiles stage short windows (rows of kTileRow samples; longer
 // windows take several passes); otherwise rows hold whole traces (rounded up
 // to 16 bytes).  tc is what fits the budget, at least 1.
-constexpr int kTileRow = 256;
+constexpr int kTileRow = kTileCols >= 8 ? 128 : 256;
 struct FwdLayout { int tc, wcap; size_t lds; };
 FwdLayout fwd_layout(const KArgs& a, int V, int fb, size_t sbytes, int tc_kernel, size_t budget) {
   const int e = (int)(16 / sbytes);
@@ -981,9 +983,9 @@ FwdLayout fwd_layout(const KArgs& a, int V, int fb, size_t sbytes, int tc_kernel
 }
 
 // LDS budget of a forward workgroup (measured on cfg1-sized volumes):
-// single-frame blocks run best at 40 KB (4 workgroups per CU, 6 tilts per
-// chunk), multi-frame blocks at 48 KB.
-constexpr size_t kFwdLds = 48 * 1024, kFwdLds1 = 40 * 1024;
+// single-frame blocks run best at 32 KB (6 tilts per chunk), multi-frame
+// blocks at 48 KB.
+constexpr size_t kFwdLds = 48 * 1024, kFwdLds1 = 32 * 1024;
 
 template <typename S> struct SName;
 
