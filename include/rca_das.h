@@ -170,10 +170,10 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_bl
 
 /*
  * Launch-shape override for tuning sweeps (no reference counterpart):
- * voxels per lane v in {1,2,4}, frames per workgroup fb in {1,2,4,8}, adjoint
- * voxel splits, LDS budget of a forward workgroup in KiB (16..64, which sets
- * how many tilts share one pass over the receive lines); 0 restores the
- * automatic choice.  Process-wide.
+ * voxels per lane v in {1,2,4}, frames per workgroup fb in {1,2,4,8} (forward
+ * and adjoint), split: reserved (>= 0, ignored), LDS budget of a workgroup in
+ * KiB (16..64, which sets how many tilts share one pass over the receive
+ * lines); 0 restores the automatic choice.  Process-wide.
  */
 int rca_set_tuning(int v, int fb, int split, int lds_kib);
 

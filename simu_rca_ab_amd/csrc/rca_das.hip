@@ -78,7 +78,7 @@ struct KArgs {
   float apx, apy, apz, fnum;
   int n_pix, n_ch, n_tx, n_sample, n_frame;
   int receive_x, mask_tx, ab_mode, compound, accumulate, vec_ok, nosum;
-  int gout_tx, n_split, atomic_flush;
+  int gout_tx;
   int depth_run, wcap;   // voxel tiling hint; LDS samples per row
   int tc;                // tilts per chunk of the forward kernel
 };
@@ -318,32 +318,35 @@ __device__ __forceinline__ void ab_terms(const KArgs& a, int t, int c, double& u
 }
 
 // ----------------------------------------------------------------------------
-// Forward kernel.
+// Voxel tiles.
 //
-// One workgroup: 256 lanes x V voxels x FB frames (fwd_block below has the
-// loop nest).  Voxel tiles: with a depth-run hint (the grid's Nz, voxel
-// p = iz + Nz*column) a workgroup owns kTileDepth depths x kTileCols*V
-// columns, so the sample window it stages per (tilt, line) is short; each
-// wave holds 64/kTileCols depths of kTileCols neighbouring columns, so
-// neighbouring lanes read nearby (often equal) samples, which spreads the LDS
-// reads over the banks.  Without the hint a workgroup owns 256V consecutive
-// voxels.
+// A workgroup has 256 lanes x V voxels.  With a depth-run hint (the grid's
+// Nz, voxel p = iz + Nz*column) it owns a tile of neighbouring depths and
+// columns, so the sample window of its voxels per (tilt, line) is short; each
+// wave holds 64/CW depths of CW neighbouring columns.  Without the hint a
+// workgroup owns 256V consecutive voxels.  The forward kernel stacks its 4
+// waves in depth with CW = kTileCols (16 depths x 16 columns): neighbouring
+// lanes read nearby (often equal) samples, which spreads its LDS reads over
+// the banks.  The adjoint puts one column in each wave (64 depths x 4
+// columns), so the lanes of a wave scatter to distinct samples.
 // ----------------------------------------------------------------------------
 #ifndef RCA_TILE_COLS
 #define RCA_TILE_COLS 16   // measured best of 4, 8, 16 on cfg1-sized volumes
 #endif
-constexpr int kTileCols = RCA_TILE_COLS;              // columns per wave
-constexpr int kWaveDepth = 64 / kTileCols;            // depths per wave
-constexpr int kTileDepth = 4 * kWaveDepth;            // depths per workgroup
+constexpr int kTileCols = RCA_TILE_COLS;   // forward: columns per wave (waves stacked in depth)
 
+// Tile of waves: CW columns per wave, DW of the 4 waves stacked in depth
+// (4/DW side by side in columns).  Tile = DW*64/CW depths x CW*(4/DW)*V columns.
+template <int CW, int DW>
 __device__ __forceinline__ long long voxel_of(const KArgs& a, unsigned bx, int V, int v, int tid,
                                               bool& valid) {
+  constexpr int WD = 64 / CW, TD = DW * WD, WC = 4 / DW;
   if (a.depth_run >= 64) {
-    const long long ndc = (a.depth_run + kTileDepth - 1) / kTileDepth;
+    const long long ndc = (a.depth_run + TD - 1) / TD;
     const long long dc = bx % ndc, cg = bx / ndc;
-    const int l = tid & 63;
-    const long long col = cg * (kTileCols * V) + v * kTileCols + (l % kTileCols);
-    const long long dep = dc * kTileDepth + (tid >> 6) * kWaveDepth + l / kTileCols;
+    const int l = tid & 63, w = tid >> 6;
+    const long long col = cg * (CW * WC * V) + v * (CW * WC) + (w / DW) * CW + (l % CW);
+    const long long dep = dc * TD + (w % DW) * WD + l / CW;
     const long long p = dep + (long long)a.depth_run * col;
     valid = dep < a.depth_run && p < a.n_pix;
     return p;
@@ -353,13 +356,18 @@ __device__ __forceinline__ long long voxel_of(const KArgs& a, unsigned bx, int V
   return p;
 }
 
-__host__ __device__ inline long long fwd_blocks(long long n_pix, int depth_run, int V) {
+template <int CW, int DW>
+__host__ __device__ inline long long tile_blocks(long long n_pix, int depth_run, int V) {
+  constexpr int TD = DW * (64 / CW), TC = CW * (4 / DW);
   if (depth_run >= 64) {
     const long long ncol = (n_pix + depth_run - 1) / depth_run;
-    return ((ncol + kTileCols * V - 1) / (kTileCols * V)) *
-           ((depth_run + kTileDepth - 1) / kTileDepth);
+    return ((ncol + TC * V - 1) / (TC * V)) * ((depth_run + TD - 1) / TD);
   }
   return (n_pix + kThreads * V - 1) / (kThreads * V);
+}
+
+__host__ __device__ inline long long fwd_blocks(long long n_pix, int depth_run, int V) {
+  return tile_blocks<kTileCols, 4>(n_pix, depth_run, V);
 }
 
 // Values every lane of the workgroup holds alike, moved to scalar registers.
@@ -409,6 +417,154 @@ struct alignas(16) TxTerm {
   double bu;   // transmit sample offset fs*((|P-VS| - d0)/c + t0)
   float2 et;   // e^{+j 2 pi frac(Bp)}, Bp = fd*((|P-VS| - d0 - 2z)/c + t0)
 };
+
+// Coordinates of a workgroup's voxels and their bounding box.
+struct TileBox {
+  float lat_lo, lat_hi;   // receive-axis coordinate
+  float txl_lo, txl_hi;   // transmit-axis coordinate
+  float zlo, zhi;         // depth
+  double apmax;           // largest receive aperture half-width z/F
+};
+
+template <int V, int CW, int DW>
+__device__ __forceinline__ TileBox tile_voxels(const KArgs& a, unsigned bx, int tid, bool rx,
+                                               double* pl, double* pt, double* pz, double* apF,
+                                               double* iapF, bool* valid, float (*s_box)[6]) {
+  float bb[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const long long p = voxel_of<CW, DW>(a, bx, V, v, tid, valid[v]);
+    const long long pc = valid[v] ? p : 0;
+    const float fx = a.gx[pc], fy = a.gy[pc], fz = a.gz[pc];
+    pl[v] = rx ? fx : fy;
+    pt[v] = rx ? fy : fx;
+    pz[v] = fz;
+    apF[v] = pz[v] / (double)a.fnum;   // .cu:174
+    iapF[v] = (double)a.fnum / pz[v];
+    if (valid[v]) {
+      bb[0] = fminf(bb[0], fx); bb[1] = fmaxf(bb[1], fx);
+      bb[2] = fminf(bb[2], fy); bb[3] = fmaxf(bb[3], fy);
+      bb[4] = fminf(bb[4], fz); bb[5] = fmaxf(bb[5], fz);
+    }
+  }
+  // wave shuffles, then LDS
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float y = __shfl_xor(bb[k], o, 64);
+      bb[k] = (k & 1) ? fmaxf(bb[k], y) : fminf(bb[k], y);
+    }
+  }
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s_box[tid >> 6][k] = bb[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    float r = s_box[0][k];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) r = (k & 1) ? fmaxf(r, s_box[w][k]) : fminf(r, s_box[w][k]);
+    bb[k] = uni(r);
+  }
+  TileBox b;
+  b.lat_lo = rx ? bb[0] : bb[2];
+  b.lat_hi = rx ? bb[1] : bb[3];
+  b.txl_lo = rx ? bb[2] : bb[0];
+  b.txl_hi = rx ? bb[3] : bb[1];
+  b.zlo = bb[4];
+  b.zhi = bb[5];
+  b.apmax = uni(a.fnum > 0.0f ? (double)b.zhi / (double)a.fnum : (double)b.zlo / (double)a.fnum);
+  return b;
+}
+
+// Transmit terms of tilts [t0, t0 + nt) for the lane's voxels into the LDS
+// cache ([j][v][lane]); bit j of act[v] = tilt t0 + j insonifies voxel v
+// (.cu:191-214).
+template <int V>
+__device__ __forceinline__ void chunk_tx(const KArgs& a, int t0, int nt, bool rx, const double* pl,
+                                         const double* pt, const double* pz, const bool* valid,
+                                         TxTerm* txc, int tid, unsigned* act) {
+#pragma unroll
+  for (int v = 0; v < V; ++v) act[v] = 0u;
+  for (int j = 0; j < nt; ++j) {
+    const int t = t0 + j;
+    const float vxf = a.vx[t], vyf = a.vy[t], vzf = a.vz[t];
+    const double d0 = a.d0[t];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      if (!valid[v]) continue;
+      // .cu:256-269: |P - VS| with the receive-axis offset dropped
+      const double dl = pt[v] - (double)(rx ? vyf : vxf), dz = pz[v] - (double)vzf;
+      const double tt = sqrt_nr(fma(dl, dl, dz * dz)) - d0;
+      const double Bp = fma(a.kfd, fma(-2.0, pz[v], tt), a.pt0);
+      float sn, cs;
+      sincos2pi((float)(Bp - rint(Bp)), sn, cs);
+      TxTerm x;
+      x.bu = fma(a.kfs, tt, a.ut0);
+      x.et = make_float2(cs, sn);
+      txc[(j * V + v) * kThreads + tid] = x;
+      const float fx = (float)(rx ? pl[v] : pt[v]), fy = (float)(rx ? pt[v] : pl[v]);
+      if (!a.mask_tx || tx_mask(a, vxf, vyf, vzf, fx, fy, (float)pz[v])) act[v] |= 1u << j;
+    }
+  }
+}
+
+// Range of the transmit sample offset over the box for tilts [t0, t0 + nt)
+// (lane L < nt takes tilt t0 + L, then a wave reduction).
+__device__ __forceinline__ void chunk_tx_bounds(const KArgs& a, const TileBox& b, int t0, int nt,
+                                                int lane, bool rx, float& bu_lo, float& bu_hi) {
+  bu_lo = INFINITY;
+  bu_hi = -INFINITY;
+  if (lane < nt) {
+    const int t = t0 + lane;
+    tx_bounds(a, b.txl_lo, b.txl_hi, b.zlo, b.zhi, rx ? a.vy[t] : a.vx[t], a.vz[t], a.d0[t], bu_lo,
+              bu_hi);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    bu_lo = fminf(bu_lo, __shfl_xor(bu_lo, o, 64));
+    bu_hi = fmaxf(bu_hi, __shfl_xor(bu_hi, o, 64));
+  }
+  bu_lo = uni(bu_lo);
+  bu_hi = uni(bu_hi);
+}
+
+// Sample window [s0, whi) of line c for the chunk's tilts over the box: taps
+// floor(u) - LO .. floor(u) + HI, one sample of margin on each side for
+// rounding, s0 a multiple of E.  Returns false when the line's aperture
+// misses every voxel (min |lateral| >= max z/F): such a line contributes
+// nothing.
+template <int LO, int HI, int E>
+__device__ __forceinline__ bool line_window(const KArgs& a, const TileBox& b, double e_lat,
+                                            double ezd, double uabc, float bu_lo, float bu_hi,
+                                            bool ab_tc, int t0, int nt, int c, int lane, int& s0,
+                                            int& whi) {
+  double A0, A1, Z0, Z1;
+  abs_range((double)b.lat_lo - e_lat, (double)b.lat_hi - e_lat, A0, A1);
+  abs_range((double)b.zlo - ezd, (double)b.zhi - ezd, Z0, Z1);
+  if (!(A0 < b.apmax)) return false;
+  A1 = fmin(A1, b.apmax);
+  float r_lo = (float)(a.kfs * sqrt(A0 * A0 + Z0 * Z0) + uabc);
+  float r_hi = (float)(a.kfs * sqrt(A1 * A1 + Z1 * Z1) + uabc);
+  if (ab_tc) {   // per-tilt delays of the TX_CHANNEL law: their range over the chunk
+    float lo = INFINITY, hi = -INFINITY;
+    if (lane < nt) lo = hi = (float)(a.fs * a.ab[(long long)(t0 + lane) * a.n_ch + c]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = fminf(lo, __shfl_xor(lo, o, 64));
+      hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+    }
+    r_lo += uni(lo);
+    r_hi += uni(hi);
+  }
+  const int ns = a.n_sample;
+  const int w_lo = (int)fminf(fmaxf(0.0f, floorf(bu_lo + r_lo) - (float)(LO + 1)), (float)ns);
+  whi = uni((int)fmaxf(fminf((float)ns, floorf(bu_hi + r_hi) + (float)(HI + 2)), 0.0f));
+  s0 = uni((w_lo / E) * E);
+  return true;
+}
 
 // Forward DAS of one workgroup: 256 threads x V voxels, frames
 // [by*FB, by*FB + FB).  Loop order: chunks of a.tc (<= TC) tilts, then
@@ -463,49 +619,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   // receive-axis and transmit-axis coordinates, depth, aperture half-width
   double pl[V], pt[V], pz[V], apF[V], iapF[V];
   bool valid[V];
-  float bb[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
-#pragma unroll
-  for (int v = 0; v < V; ++v) {
-    const long long p = voxel_of(a, bx, V, v, tid, valid[v]);
-    const long long pc = valid[v] ? p : 0;
-    const float fx = a.gx[pc], fy = a.gy[pc], fz = a.gz[pc];
-    pl[v] = rx ? fx : fy;
-    pt[v] = rx ? fy : fx;
-    pz[v] = fz;
-    apF[v] = pz[v] / (double)a.fnum;   // .cu:174
-    iapF[v] = (double)a.fnum / pz[v];
-    if (valid[v]) {
-      bb[0] = fminf(bb[0], fx); bb[1] = fmaxf(bb[1], fx);
-      bb[2] = fminf(bb[2], fy); bb[3] = fmaxf(bb[3], fy);
-      bb[4] = fminf(bb[4], fz); bb[5] = fmaxf(bb[5], fz);
-    }
-  }
-  // bounding box of the workgroup's voxels (wave shuffles, then LDS)
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const float y = __shfl_xor(bb[k], o, 64);
-      bb[k] = (k & 1) ? fmaxf(bb[k], y) : fminf(bb[k], y);
-    }
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) s_box[tid >> 6][k] = bb[k];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    float r = s_box[0][k];
-#pragma unroll
-    for (int w = 1; w < 4; ++w) r = (k & 1) ? fmaxf(r, s_box[w][k]) : fminf(r, s_box[w][k]);
-    bb[k] = uni(r);
-  }
-  // receive-axis, transmit-axis and depth ranges
-  const float lat_lo = rx ? bb[0] : bb[2], lat_hi = rx ? bb[1] : bb[3];
-  const float txl_lo = rx ? bb[2] : bb[0], txl_hi = rx ? bb[3] : bb[1];
-  const float zlo = bb[4], zhi = bb[5];
-  const double apmax = uni(a.fnum > 0.0f ? (double)zhi / (double)a.fnum : (double)zlo / (double)a.fnum);
+  const TileBox box = tile_voxels<V, kTileCols, 4>(a, bx, tid, rx, pl, pt, pz, apF, iapF, valid, s_box);
 
   float2 acc[V][FB];
   float macc[V];
@@ -518,46 +632,10 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
 
   for (int t0 = 0; t0 < a.n_tx; t0 += tcn) {
     const int nt = min(tcn, a.n_tx - t0);
-    // transmit terms of the chunk's tilts, per voxel, into the cache; the
-    // tilts that insonify each voxel (.cu:191-214) as bits
     unsigned act[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) act[v] = 0u;
-    for (int j = 0; j < nt; ++j) {
-      const int t = t0 + j;
-      const float vxf = a.vx[t], vyf = a.vy[t], vzf = a.vz[t];
-      const double d0 = a.d0[t];
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        if (!valid[v]) continue;
-        // .cu:256-269: |P - VS| with the receive-axis offset dropped
-        const double dl = pt[v] - (double)(rx ? vyf : vxf), dz = pz[v] - (double)vzf;
-        const double tt = sqrt_nr(fma(dl, dl, dz * dz)) - d0;
-        const double Bp = fma(a.kfd, fma(-2.0, pz[v], tt), a.pt0);
-        float sn, cs;
-        sincos2pi((float)(Bp - rint(Bp)), sn, cs);
-        TxTerm x;
-        x.bu = fma(a.kfs, tt, a.ut0);
-        x.et = make_float2(cs, sn);
-        txc[(j * V + v) * kThreads + tid] = x;
-        const float fx = (float)(rx ? pl[v] : pt[v]), fy = (float)(rx ? pt[v] : pl[v]);
-        if (!a.mask_tx || tx_mask(a, vxf, vyf, vzf, fx, fy, (float)pz[v])) act[v] |= 1u << j;
-      }
-    }
-    // transmit sample-offset bounds of the chunk's tilts over the box (lane
-    // L < nt takes tilt t0 + L, then a wave reduction)
-    float bu_lo = INFINITY, bu_hi = -INFINITY;
-    if (lane < nt) {
-      const int t = t0 + lane;
-      tx_bounds(a, txl_lo, txl_hi, zlo, zhi, rx ? a.vy[t] : a.vx[t], a.vz[t], a.d0[t], bu_lo, bu_hi);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      bu_lo = fminf(bu_lo, __shfl_xor(bu_lo, o, 64));
-      bu_hi = fmaxf(bu_hi, __shfl_xor(bu_hi, o, 64));
-    }
-    bu_lo = uni(bu_lo);
-    bu_hi = uni(bu_hi);
+    chunk_tx<V>(a, t0, nt, rx, pl, pt, pz, valid, txc, tid, act);
+    float bu_lo, bu_hi;
+    chunk_tx_bounds(a, box, t0, nt, lane, rx, bu_lo, bu_hi);
 
     float2 accT[V][FB];
     float mT[V];
@@ -577,38 +655,15 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
         uabc = a.fs * ab;
         pabc = a.fd * ab;
       }
-      // receive bounds of line c over the box; a line whose aperture misses
-      // every voxel (min |lateral| >= max z/F) contributes nothing
-      double A0, A1, Z0, Z1;
-      abs_range((double)lat_lo - e_lat, (double)lat_hi - e_lat, A0, A1);
-      abs_range((double)zlo - ezd, (double)zhi - ezd, Z0, Z1);
-      const bool any_ap = A0 < apmax;
       if (!CMP && a.nosum) {   // per-line outputs: the sums restart at every line
 #pragma unroll
         for (int v = 0; v < V; ++v)
 #pragma unroll
           for (int fi = 0; fi < FB; ++fi) accT[v][fi] = make_float2(0.0f, 0.0f);
       }
-      if (any_ap) {
-        A1 = fmin(A1, apmax);
-        float r_lo = (float)(a.kfs * sqrt(A0 * A0 + Z0 * Z0) + uabc);
-        float r_hi = (float)(a.kfs * sqrt(A1 * A1 + Z1 * Z1) + uabc);
-        if (ab_tc) {   // per-tilt delays of the TX_CHANNEL law: their range over the chunk
-          float lo = INFINITY, hi = -INFINITY;
-          if (lane < nt) lo = hi = (float)(a.fs * a.ab[(long long)(t0 + lane) * a.n_ch + c]);
-#pragma unroll
-          for (int o = 32; o > 0; o >>= 1) {
-            lo = fminf(lo, __shfl_xor(lo, o, 64));
-            hi = fmaxf(hi, __shfl_xor(hi, o, 64));
-          }
-          r_lo += uni(lo);
-          r_hi += uni(hi);
-        }
-        // one window [s0, whi) for all the chunk's tilts: taps floor(u) - LO ..
-        // floor(u) + HI, with one sample of margin on each side for rounding
-        const int w_lo = (int)fminf(fmaxf(0.0f, floorf(bu_lo + r_lo) - (float)(LO + 1)), (float)ns);
-        const int whi = uni((int)fmaxf(fminf((float)ns, floorf(bu_hi + r_hi) + (float)(HI + 2)), 0.0f));
-        const int s0 = uni((w_lo / E) * E);
+      int s0, whi;
+      if (line_window<LO, HI, E>(a, box, e_lat, ezd, uabc, bu_lo, bu_hi, ab_tc, t0, nt, c, lane, s0,
+                                 whi)) {
         const int npass = whi - s0 > wcap ? 1 + (whi - s0 - wcap + step - 1) / step : 1;
 
         for (int k = 0; k < npass; ++k) {
@@ -723,7 +778,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           if (!valid[v]) continue;
-          const long long p = voxel_of(a, bx, V, v, tid, ok);
+          const long long p = voxel_of<kTileCols, 4>(a, bx, V, v, tid, ok);
 #pragma unroll
           for (int fi = 0; fi < FB; ++fi) {
             if (fi < nf) {
@@ -741,7 +796,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         if (!valid[v]) continue;
-        const long long p = voxel_of(a, bx, V, v, tid, ok);
+        const long long p = voxel_of<kTileCols, 4>(a, bx, V, v, tid, ok);
         if (!a.nosum) {
 #pragma unroll
           for (int fi = 0; fi < FB; ++fi) {
@@ -765,7 +820,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       if (!valid[v]) continue;
-      const long long p = voxel_of(a, bx, V, v, tid, ok);
+      const long long p = voxel_of<kTileCols, 4>(a, bx, V, v, tid, ok);
 #pragma unroll
       for (int fi = 0; fi < FB; ++fi) {
         if (fi < nf) {
@@ -839,78 +894,246 @@ __global__ void __launch_bounds__(kThreads) das_fwd_batch(const int* __restrict_
 }
 
 // ----------------------------------------------------------------------------
-// Adjoint kernel: one block per (line, tilt, voxel split, frame block).
+// Adjoint kernel (the transpose of fwd_block): 256 lanes, one voxel each,
+// frames [by*FB, by*FB + FB).  Same depth tiles (one column per wave: the 64
+// lanes of a wave hold consecutive depths about two samples apart), tilt
+// chunks, transmit cache and per-line windows as the forward.  For every
+// update the lane adds q_k * conj(w e^{j2pi p}) * g to taps m-LO .. m+HI of its
+// tilt's window rows in LDS with LDS float atomics (real and imaginary parts
+// in separate planes, so a wave's atomics spread over the banks).  After each
+// line (and pass) the rows are added to the I/Q gradient with global float
+// atomics: the tiles that share samples of a trace all add to them, so the
+// gradient must start at zero (the host clears it unless accumulating).
 // ----------------------------------------------------------------------------
-template <int KIND, int V, int FB>
-__global__ void __launch_bounds__(kThreads) das_adj(const KArgs a) {
+// One voxel's adjoint terms for the chunk's tilts on one line: for tilt j
+// with an in-range sample position, q_k * conj(w e^{j2pi p}) * g added to taps
+// m-LO .. m+HI of the wave's row [j][fi] (rows hold samples [cb, cb + wcap),
+// real plane then imaginary plane).  PT: per-tilt output gradients, read from
+// memory; otherwise g holds the voxel's compounded gradient per frame.
+//
+// The rows are private to the wave, so only the wave's own lanes can touch
+// the same sample.  Lanes of one wave hold different depths, whose first taps
+// normally differ: then the taps are added with plain LDS read-modify-writes,
+// one tap after another in program order (a compiler barrier keeps that
+// order; LDS executes a wave's instructions in order), so a lane's tap k+d
+// that lands on another lane's tap k sees that lane's write.  When two active
+// lanes share a first tap (a claim slot per first tap detects it) the tilt's
+// taps go through LDS float atomics instead, which are slower but safe.
+template <int KIND, int FB, bool PT>
+__device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, float* rw,
+                                            int* claim, unsigned act, int nt, float er, float ei,
+                                            double ur, int cb, bool last, int step, int k,
+                                            int wcap, int nf, const float2* g, int f0, int t0,
+                                            long long p, bool ab_tc, const double* s_uab,
+                                            const float2* s_eab) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  float* acc = reinterpret_cast<float*>(smem_raw);   // [FB][ns][2]
-  const int c = blockIdx.x, t = blockIdx.y;
-  const int split = blockIdx.z % a.n_split;
-  const int f0 = (blockIdx.z / a.n_split) * FB;
-  const int nf = min(FB, a.n_frame - f0);
   const int ns = a.n_sample;
-  const int tid = threadIdx.x;
-
-  for (int i = tid; i < FB * ns * 2; i += kThreads) acc[i] = 0.0f;
-  __syncthreads();
-
-  const long long per = ((long long)a.n_pix + a.n_split - 1) / a.n_split;
-  const long long p0 = per * split;
-  const long long p1 = min((long long)a.n_pix, p0 + per);
-  const float vxf = a.vx[t], vyf = a.vy[t], vzf = a.vz[t];
-  const double d0 = a.d0[t];
-  const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
-  double uab, pab;
-  ab_terms(a, t, c, uab, pab);
-  const int gt = a.gout_tx == 1 ? 0 : t;
-
-  for (long long pb = p0; pb < p1; pb += (long long)kThreads * V) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll 1
+  for (int j = 0; j < nt; ++j, tv += kThreads, rw += FB * 2 * wcap) {
+    if (!((act >> j) & 1u)) continue;
+    const TxTerm x = *tv;
+    double u = x.bu + ur;
+    if (ab_tc) u += s_uab[j];
+    const double mf = floor(u);                                       // .cu:319
+    if (!(mf >= (double)LO && mf < (double)(ns - 1 - HI))) continue;  // .cu:323
+    const int base = (int)mf - LO;
+    if ((k > 0 && base < cb) || (!last && base >= cb + step)) continue;
+    float q[NT];
+    interp_weights<KIND>((float)(u - mf), q);
+    float pr = er * x.et.x - ei * x.et.y;
+    float pi = er * x.et.y + ei * x.et.x;
+    if (ab_tc) {
+      const float2 eab = s_eab[j];
+      const float r2 = pr * eab.x - pi * eab.y;
+      pi = pr * eab.y + pi * eab.x;
+      pr = r2;
+    }
+    // the bounds put every tap inside the row; the clamp only keeps LDS
+    // accesses inside it
+    const int off = min(max(base - cb, 0), wcap - NT);
+    float* rv = rw + off;
+    float br[FB], bi[FB];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const long long p = pb + (long long)v * kThreads + tid;
-      if (p >= p1) continue;
-      const double px = a.gx[p], py = a.gy[p], pz = a.gz[p];
-      if (a.mask_tx && !tx_mask(a, vxf, vyf, vzf, (float)px, (float)py, (float)pz)) continue;
-      double Bu, Bp;
-      tx_terms(a, px, py, pz, vxf, vyf, vzf, d0, Bu, Bp);
-      const Upd g = update_geom<LO, HI>(a, px, py, pz, pz / (double)a.fnum,
-                                        (double)a.fnum / pz, Bu, Bp, exd, eyd, ezd, uab, pab);
-      if (!g.in_ap || !g.in_rng) continue;
-      float q[NT];
-      interp_weights<KIND>(g.d, q);
-      float sn, cs;
-      sincos2pi(g.p, sn, cs);
-      const float wr = g.w * cs, wi = g.w * sn;
+    for (int fi = 0; fi < FB; ++fi) {
+      float2 gg = make_float2(0.0f, 0.0f);
+      if (fi < nf) {
+        if constexpr (PT) gg = a.gout[((long long)(f0 + fi) * a.n_tx + t0 + j) * a.n_pix + p];
+        else gg = g[fi];
+      }
+      // conj(w e^{j 2 pi p}) * g
+      br[fi] = fmaf(gg.x, pr, gg.y * pi);
+      bi[fi] = fmaf(gg.y, pr, -gg.x * pi);
+    }
+    // duplicate first taps among the active lanes?
+    claim[off] = lane;
+    __asm__ volatile("" ::: "memory");
+    const bool dup = __ballot(claim[off] != lane) != 0;
+    if (!dup) {
+#pragma unroll
+      for (int kk = 0; kk < NT; ++kk) {
+        float vr[FB], vi[FB];
+#pragma unroll
+        for (int fi = 0; fi < FB; ++fi) {
+          vr[fi] = rv[fi * 2 * wcap + kk];
+          vi[fi] = rv[fi * 2 * wcap + wcap + kk];
+        }
+#pragma unroll
+        for (int fi = 0; fi < FB; ++fi) {
+          rv[fi * 2 * wcap + kk] = fmaf(q[kk], br[fi], vr[fi]);
+          rv[fi * 2 * wcap + wcap + kk] = fmaf(q[kk], bi[fi], vi[fi]);
+        }
+        __asm__ volatile("" ::: "memory");
+      }
+    } else {
 #pragma unroll
       for (int fi = 0; fi < FB; ++fi) {
-        if (fi < nf) {
-          const float2 gg = a.gout[((long long)(f0 + fi) * a.gout_tx + gt) * a.n_pix + p];
-          // conj(w e^{j 2 pi p}) * g
-          const float br = fmaf(gg.x, wr, gg.y * wi);
-          const float bi = fmaf(gg.y, wr, -gg.x * wi);
-          float* dst = acc + 2 * (fi * ns + g.m - LO);
 #pragma unroll
-          for (int k = 0; k < NT; ++k) {
-            atomicAdd(dst + 2 * k, q[k] * br);
-            atomicAdd(dst + 2 * k + 1, q[k] * bi);
+        for (int kk = 0; kk < NT; ++kk) {
+          atomicAdd(rv + fi * 2 * wcap + kk, q[kk] * br[fi]);
+          atomicAdd(rv + fi * 2 * wcap + wcap + kk, q[kk] * bi[fi]);
+        }
+      }
+    }
+    __asm__ volatile("" ::: "memory");
+  }
+}
+
+template <int KIND, int FB, int TC>
+__device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned by,
+                                          unsigned char* smem_raw) {
+  constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
+  constexpr int R = TC * FB;          // rows per wave at most
+  constexpr int G = kThreads / R;     // threads per row when clearing and flushing
+  static_assert(R <= 64 && kThreads % R == 0, "rows must divide the workgroup");
+  TxTerm* txc = reinterpret_cast<TxTerm*>(smem_raw);
+  __shared__ float s_box[4][6];
+  __shared__ double s_uab[TC];        // TX_CHANNEL law: per-tilt delay (samples)
+  __shared__ float2 s_eab[TC];        // and phasor
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int f0 = by * FB;
+  const int nf = min(FB, a.n_frame - f0);
+  const int ns = a.n_sample;
+  const int wcap = a.wcap;
+  const int tcn = a.tc;   // tilts per chunk (<= TC)
+  // per wave: rows [j][fi][re|im][wcap]; then per wave a claim slot per sample
+  float* rows = reinterpret_cast<float*>(smem_raw + (size_t)tcn * kThreads * sizeof(TxTerm));
+  const size_t wrows = (size_t)tcn * FB * 2 * wcap;   // floats per wave
+  float* myrows = rows + wave * wrows;
+  int* claim = reinterpret_cast<int*>(rows + 4 * wrows) + wave * wcap;
+  const int step = max(wcap - NT + 1, 1);   // pass advance
+  const bool rx = a.receive_x != 0;
+  const bool ab_tc = a.ab_mode == RCA_ABLAW_TX_CHANNEL;
+  const bool gt1 = a.gout_tx == 1;
+
+  double pl[1], pt[1], pz[1], apF[1], iapF[1];
+  bool valid[1];
+  const TileBox box = tile_voxels<1, 1, 1>(a, bx, tid, rx, pl, pt, pz, apF, iapF, valid, s_box);
+  bool ok;
+  const long long p = voxel_of<1, 1>(a, bx, 1, 0, tid, ok);
+  // the voxel's output gradient per frame (compounded output)
+  float2 g[FB];
+#pragma unroll
+  for (int fi = 0; fi < FB; ++fi)
+    g[fi] = (valid[0] && gt1 && fi < nf) ? a.gout[(long long)(f0 + fi) * a.n_pix + p]
+                                         : make_float2(0.0f, 0.0f);
+  const int row = tid / G, li = tid % G;   // clearing / flushing assignment
+  const int rj = row / FB, rfi = row % FB;
+
+  for (int t0 = 0; t0 < a.n_tx; t0 += tcn) {
+    const int nt = min(tcn, a.n_tx - t0);
+    unsigned act[1];
+    chunk_tx<1>(a, t0, nt, rx, pl, pt, pz, valid, txc, tid, act);
+    float bu_lo, bu_hi;
+    chunk_tx_bounds(a, box, t0, nt, lane, rx, bu_lo, bu_hi);
+
+    for (int c = 0; c < a.n_ch; ++c) {
+      const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
+      const double e_lat = rx ? exd : eyd;
+      double uabc = 0.0, pabc = 0.0;
+      if (a.ab_mode == RCA_ABLAW_CHANNEL) {
+        const double ab = a.ab[c];
+        uabc = a.fs * ab;
+        pabc = a.fd * ab;
+      }
+      int s0, whi;
+      if (!line_window<LO, HI, 1>(a, box, e_lat, ezd, uabc, bu_lo, bu_hi, ab_tc, t0, nt, c, lane,
+                                  s0, whi))
+        continue;
+      const int npass = whi - s0 > wcap ? 1 + (whi - s0 - wcap + step - 1) / step : 1;
+
+      for (int k = 0; k < npass; ++k) {
+        const int cb = s0 + k * step;
+        const bool last = cb + wcap >= whi;
+        const int len = min(cb + wcap, whi) - cb;
+        // clear every wave's rows [j][fi] (the previous flush has read them),
+        // and publish the TX_CHANNEL law's per-tilt delays and phasors
+        __syncthreads();
+        if (rj < nt && rfi < nf) {
+          for (int w = 0; w < 4; ++w) {
+            float* r = rows + w * wrows + (size_t)row * 2 * wcap;
+            for (int i = li; i < len; i += G) {
+              r[i] = 0.0f;
+              r[wcap + i] = 0.0f;
+            }
+          }
+        }
+        if (ab_tc && k == 0 && tid < nt) {
+          const double ab = a.ab[(long long)(t0 + tid) * a.n_ch + c];
+          const double pab = a.fd * ab;
+          float sn, cs;
+          sincos2pi((float)(pab - rint(pab)), sn, cs);
+          s_uab[tid] = a.fs * ab;
+          s_eab[tid] = make_float2(cs, sn);
+        }
+        __syncthreads();
+
+        const double A = fabs(pl[0] - e_lat);   // .cu:284-291
+        if (act[0] && A < apF[0]) {             // .cu:294
+          const float w = hann((float)(A * iapF[0]));   // .cu:299
+          const double dz = pz[0] - ezd;
+          const double Rr = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));   // .cu:308-309
+          const double ur = fma(a.kfs, Rr, uabc);                         // .cu:310-311
+          const double ph = fma(a.kfd, Rr, pabc);                         // .cu:371-373
+          float sn, cs;
+          sincos2pi((float)(ph - rint(ph)), sn, cs);
+          const float er = w * cs, ei = w * sn;
+          if (gt1)
+            adj_scatter<KIND, FB, false>(a, txc + tid, myrows, claim, act[0], nt, er, ei, ur, cb,
+                                         last, step, k, wcap, nf, g, f0, t0, p, ab_tc, s_uab,
+                                         s_eab);
+          else
+            adj_scatter<KIND, FB, true>(a, txc + tid, myrows, claim, act[0], nt, er, ei, ur, cb,
+                                        last, step, k, wcap, nf, g, f0, t0, p, ab_tc, s_uab,
+                                        s_eab);
+        }
+        __syncthreads();
+        // add the four waves' rows to the gradient traces (other tiles add to
+        // the same samples)
+        if (rj < nt && rfi < nf) {
+          const float* r = rows + (size_t)row * 2 * wcap;
+          float* dst = reinterpret_cast<float*>(
+              a.gin + (((long long)(f0 + rfi) * a.n_tx + t0 + rj) * a.n_ch + c) * ns + cb);
+          for (int i = li; i < len; i += G) {
+            const float vr = r[i] + r[wrows + i] + r[2 * wrows + i] + r[3 * wrows + i];
+            const float vi = r[wcap + i] + r[wrows + wcap + i] + r[2 * wrows + wcap + i] +
+                             r[3 * wrows + wcap + i];
+            if (vr != 0.0f) unsafeAtomicAdd(dst + 2 * i, vr);
+            if (vi != 0.0f) unsafeAtomicAdd(dst + 2 * i + 1, vi);
           }
         }
       }
     }
   }
   __syncthreads();
+}
 
-  for (int fi = 0; fi < nf; ++fi) {
-    float* gdst = reinterpret_cast<float*>(
-        a.gin + (((long long)(f0 + fi) * a.n_tx + t) * a.n_ch + c) * ns);
-    const float* src = acc + 2 * fi * ns;
-    for (int i = tid; i < 2 * ns; i += kThreads) {
-      if (a.atomic_flush) unsafeAtomicAdd(gdst + i, src[i]);
-      else gdst[i] = a.accumulate ? gdst[i] + src[i] : src[i];
-    }
-  }
+template <int KIND, int FB>
+__global__ void __launch_bounds__(kThreads) das_adj(const KArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  adj_block<KIND, FB, tc_max(FB, 8)>(a, blockIdx.x, blockIdx.y, smem_raw);
 }
 
 __global__ void fill_zero(float* __restrict__ p, long long n) {
@@ -921,7 +1144,7 @@ __global__ void fill_zero(float* __restrict__ p, long long n) {
 // ----------------------------------------------------------------------------
 // Host side.
 // ----------------------------------------------------------------------------
-int g_tune_v = 0, g_tune_fb = 0, g_tune_split = 0, g_tune_lds = 0;
+int g_tune_v = 0, g_tune_fb = 0, g_tune_lds = 0;
 
 int check_desc(const rca_das_desc* d) {
   if (!d) return fail(RCA_ERR_INVALID, "desc is NULL");
@@ -970,11 +1193,12 @@ KArgs make_args(const rca_das_desc* d, int n_frame) {
 // to 16 bytes).  tc is what fits the budget, at least 1.
 constexpr int kTileRow = kTileCols >= 8 ? 128 : 256;
 struct FwdLayout { int tc, wcap; size_t lds; };
-FwdLayout fwd_layout(const KArgs& a, int V, int fb, size_t sbytes, int tc_kernel, size_t budget) {
+FwdLayout fwd_layout(const KArgs& a, int V, int fb, size_t sbytes, int tc_kernel, size_t budget,
+                     int tile_row = kTileRow) {
   const int e = (int)(16 / sbytes);
   const int full = (a.n_sample + e - 1) / e * e;
   FwdLayout L;
-  L.wcap = a.depth_run && full > kTileRow ? kTileRow : full;
+  L.wcap = a.depth_run && full > tile_row ? tile_row : full;
   const size_t per_tilt = (size_t)V * kThreads * sizeof(TxTerm) + (size_t)fb * L.wcap * sbytes;
   const long long fit = (long long)(budget / per_tilt);
   L.tc = (int)(fit < 1 ? 1 : fit > tc_kernel ? tc_kernel : fit);
@@ -1048,11 +1272,38 @@ int pick_v(long long, int, int, int, bool) {
   return 1;
 }
 
-template <int KIND, int V, int FB>
-int launch_adj_t(const KArgs& a, hipStream_t st) {
-  const size_t lds = (size_t)FB * a.n_sample * 2 * sizeof(float);
-  const dim3 grid(a.n_ch, a.n_tx, a.n_split * ((a.n_frame + FB - 1) / FB));
-  hipLaunchKernelGGL((das_adj<KIND, V, FB>), grid, dim3(kThreads), lds, st, a);
+// Adjoint layout: per tilt of a chunk 256 transmit terms and, for each of the
+// 4 waves, FB rows of complex FP32 accumulators (two float planes); then a
+// claim slot per row sample for each wave.  Depth tiles (64 depths x 4
+// columns) use rows of kAdjTileRow samples, other tiles whole traces; rows
+// are capped so one tilt fits 64 KB (longer windows take several passes).
+constexpr int kAdjTileRow = 192;
+constexpr size_t kAdjLds = 48 * 1024;
+
+struct AdjLayout { int tc, wcap; size_t lds; };
+AdjLayout adj_layout(const KArgs& a, int fb, int tc_kernel, size_t budget) {
+  AdjLayout L;
+  const int w0 = a.depth_run && a.n_sample > kAdjTileRow ? kAdjTileRow : a.n_sample;
+  const int wmax = (int)((kMaxLds - kThreads * sizeof(TxTerm)) / (4 * 8 * (size_t)fb + 4 * 4));
+  L.wcap = w0 < wmax ? w0 : wmax;
+  const size_t per_tilt = kThreads * sizeof(TxTerm) + 4 * 8 * (size_t)fb * L.wcap;
+  const size_t claims = 4 * 4 * (size_t)L.wcap;
+  const long long fit = budget > claims ? (long long)((budget - claims) / per_tilt) : 0;
+  L.tc = (int)(fit < 1 ? 1 : fit > tc_kernel ? tc_kernel : fit);
+  L.lds = L.tc * per_tilt + claims;
+  return L;
+}
+
+template <int KIND, int FB>
+int launch_adj_t(KArgs a, hipStream_t st) {
+  const AdjLayout L = adj_layout(a, FB, tc_max(FB, 8), g_tune_lds ? (size_t)g_tune_lds * 1024 : kAdjLds);
+  if (L.lds > (size_t)kMaxLds || L.wcap < Interp<KIND>::LO + Interp<KIND>::HI + 1)
+    return fail(RCA_ERR_UNSUPPORTED, "FB=%d adjoint rows do not fit in LDS", FB);
+  a.tc = L.tc;
+  a.wcap = L.wcap;
+  const dim3 grid((unsigned)tile_blocks<1, 1>(a.n_pix, a.depth_run, 1),
+                  (unsigned)((a.n_frame + FB - 1) / FB));
+  hipLaunchKernelGGL((das_adj<KIND, FB>), grid, dim3(kThreads), L.lds, st, a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "das_adj launch");
 }
@@ -1060,10 +1311,10 @@ int launch_adj_t(const KArgs& a, hipStream_t st) {
 template <int KIND>
 int launch_adj_k(const KArgs& a, int FB, hipStream_t st) {
   switch (FB) {
-    case 1: return launch_adj_t<KIND, 2, 1>(a, st);
-    case 2: return launch_adj_t<KIND, 2, 2>(a, st);
-    case 4: return launch_adj_t<KIND, 2, 4>(a, st);
-    case 8: return launch_adj_t<KIND, 2, 8>(a, st);
+    case 1: return launch_adj_t<KIND, 1>(a, st);
+    case 2: return launch_adj_t<KIND, 2>(a, st);
+    case 4: return launch_adj_t<KIND, 4>(a, st);
+    case 8: return launch_adj_t<KIND, 8>(a, st);
   }
   return fail(RCA_ERR_UNSUPPORTED, "no adjoint kernel for FB=%d", FB);
 }
@@ -1084,7 +1335,6 @@ int rca_set_tuning(int v, int fb, int split, int lds_kib) {
     return fail(RCA_ERR_INVALID, "bad tuning V=%d FB=%d split=%d lds=%d KiB", v, fb, split, lds_kib);
   g_tune_v = v;
   g_tune_fb = fb;
-  g_tune_split = split;
   g_tune_lds = lds_kib;
   return RCA_OK;
 }
@@ -1149,41 +1399,23 @@ int rca_adjoint(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_
   if (n_frame < 0) return fail(RCA_ERR_INVALID, "nFrame < 0");
   if (flags & ~RCA_FLAG_ACCUMULATE) return fail(RCA_ERR_INVALID, "unknown flags 0x%x", flags);
   if (n_frame == 0) return RCA_OK;
-  if ((size_t)d->n_sample * 8 > (size_t)kMaxLds)
-    return fail(RCA_ERR_UNSUPPORTED, "nSample=%d exceeds the LDS trace capacity", d->n_sample);
   KArgs a = make_args(d, n_frame);
   a.gout = reinterpret_cast<const float2*>(grad_out);
   a.gin = reinterpret_cast<float2*>(grad_iq);
   a.gout_tx = grad_tx;
   a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
-  int fb = 8;   // adjoint: FB x nSample complex FP32 accumulators in LDS
-  while (fb > 1 && (fb > n_frame || (size_t)fb * d->n_sample * 8 > (size_t)kMaxLds)) fb >>= 1;
-  const int nfb = (n_frame + fb - 1) / fb;
-  const long long traces = (long long)d->n_channel * d->n_tx * nfb;
-  int split = g_tune_split;
-  if (!split) {
-    split = (int)((2048 + traces - 1) / traces);
-    const long long maxs = ((long long)d->n_pix + kThreads * 8 - 1) / (kThreads * 8);
-    if (split > maxs) split = (int)(maxs > 0 ? maxs : 1);
-    if (split < 1) split = 1;
-  }
-  a.n_split = split;
-  a.atomic_flush = split > 1;
-  if (a.atomic_flush && !a.accumulate) {
+  // one frame per workgroup measured fastest (more tilts per chunk fit LDS)
+  int fb = g_tune_fb ? g_tune_fb : 1;
+  while (fb > 1 && fb > n_frame) fb >>= 1;
+  // every workgroup adds its windows to the traces: start from zero
+  if (!a.accumulate) {
     const long long n = 2LL * n_frame * d->n_tx * d->n_channel * d->n_sample;
     hipLaunchKernelGGL(fill_zero, dim3(2048), dim3(256), 0, stream,
                        reinterpret_cast<float*>(grad_iq), n);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "fill_zero launch");
   }
-  if (d->n_pix == 0) {
-    if (!a.atomic_flush && !a.accumulate) {
-      const long long n = 2LL * n_frame * d->n_tx * d->n_channel * d->n_sample;
-      hipLaunchKernelGGL(fill_zero, dim3(2048), dim3(256), 0, stream,
-                         reinterpret_cast<float*>(grad_iq), n);
-    }
-    return RCA_OK;
-  }
+  if (d->n_pix == 0) return RCA_OK;
   switch (d->interp) {
     case RCA_INTERP_QUADRATIC: return launch_adj_k<RCA_INTERP_QUADRATIC>(a, fb, stream);
     case RCA_INTERP_LANCZOS_REF: return launch_adj_k<RCA_INTERP_LANCZOS_REF>(a, fb, stream);

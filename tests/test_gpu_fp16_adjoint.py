@@ -68,16 +68,16 @@ def test_adjoint_vs_oracle(dev, interp, gtx):
     assert r <= 1e-5
 
 
-def test_adjoint_split_and_accumulate(dev):
-    say("case adjoint voxel splits (atomic flush) and accumulate")
+def test_adjoint_frame_blocks_and_accumulate(dev):
+    say("case adjoint frame blocks 1 and 2 (3 frames), accumulate")
     plan, G, T = plan_for(ADJ, 0, dev, "lanczos3")
     rng = np.random.default_rng(2)
     g = torch.from_numpy((rng.standard_normal((3, 1, plan.n_pix)) +
                           1j * rng.standard_normal((3, 1, plan.n_pix))).astype(np.complex64)).to(dev)
     try:
-        _lib.set_tuning(0, 0, 1)
+        _lib.set_tuning(0, 1, 0)
         a = plan.adjoint(g)
-        _lib.set_tuning(0, 0, 5)
+        _lib.set_tuning(0, 2, 0)
         b = plan.adjoint(g)
         c = b.clone()
         plan.adjoint(g, grad_iq=c, accumulate=True)

@@ -11,16 +11,18 @@ interpolation, both halves, per-tilt and compounded output, frame counts that
 leave partial frame blocks, column counts that leave partial column groups, a
 hint that does not divide the voxel count, a depth-coarse grid whose windows
 are longer than the LDS rows (staged in several passes), several LDS budgets
-(tilts per chunk), FP16-I/Q and NoSum.
+(tilts per chunk), FP16-I/Q, NoSum, and the adjoint (tiled vs untiled, FP64
+oracle adjoint, frame blocks, per-tilt gradients, accumulation).
 """
 import numpy as np
 import pytest
 import torch
 
+from oracle import rca_oracle as O
 from simu_rca_ab_amd import _lib, configs as C
 from simu_rca_ab_amd.plan import RcaPlan, quantize_fp16
 from tests import helpers as H
-from tests.test_gpu_forward import check, oracle_fwd, say
+from tests.test_gpu_forward import OKIND, check, oracle_fwd, say
 
 pytestmark = pytest.mark.gpu
 
@@ -134,3 +136,52 @@ def test_tiled_fp16_and_nosum(dev):
     r = H.rel_l2(n_t.cpu().numpy(), n_f.cpu().numpy())
     say(f"nosum tiled vs untiled: rel-L2 {r:.3e}")
     assert r <= 1e-6
+
+
+@pytest.mark.parametrize("interp", ["quadratic", "lanczos4"])
+@pytest.mark.parametrize("half_index", [0, 1])
+def test_tiled_adjoint(dev, interp, half_index):
+    say(f"case tiled adjoint interp={interp} half={half_index}: vs untiled, oracle, frame blocks")
+    flat, tiled, G = plans(TILE, half_index, dev, interp)
+    rng = np.random.default_rng(11 + half_index)
+    nf = 3
+    g = torch.from_numpy((rng.standard_normal((nf, 1, flat.n_pix)) +
+                          1j * rng.standard_normal((nf, 1, flat.n_pix))).astype(np.complex64)).to(dev)
+    a_f = flat.adjoint(g).cpu().numpy()
+    outs = {}
+    try:
+        for fb, kib in ((0, 0), (1, 0), (2, 0), (4, 0), (8, 0), (1, 16), (4, 64)):
+            _lib.set_tuning(0, fb, 0, kib)
+            outs[(fb, kib)] = tiled.adjoint(g).cpu().numpy()
+    finally:
+        _lib.set_tuning(0, 0, 0, 0)
+    for k, o in outs.items():
+        r = H.rel_l2(o, a_f)
+        say(f"FB={k[0]} LDS={k[1]} KiB: rel-L2 vs untiled {r:.3e}")
+        assert r <= TOL, k
+    T = H.case(TILE, half_index)[1]
+    prep = O.prepare(G, T)
+    kind, a = OKIND[interp]
+    ref = O.kernel_adjoint(g.cpu().numpy(), H.grid_f32(G), prep,
+                           H.oracle_params(TILE, TILE.halves[half_index]), nf, kind, a)
+    r = H.rel_l2(outs[(0, 0)], ref)
+    say(f"tiled adjoint vs FP64 oracle: rel-L2 {r:.3e}")
+    assert r <= 1e-5
+
+
+def test_tiled_adjoint_per_tilt_and_accumulate(dev):
+    say("case tiled adjoint of per-tilt outputs (grad_tx = nTx), accumulate")
+    flat, tiled, G = plans(TILE, 0, dev, "lanczos3")
+    rng = np.random.default_rng(21)
+    g = torch.from_numpy((rng.standard_normal((2, flat.n_tx, flat.n_pix)) +
+                          1j * rng.standard_normal((2, flat.n_tx, flat.n_pix))
+                          ).astype(np.complex64)).to(dev)
+    a_f = flat.adjoint(g)
+    a_t = tiled.adjoint(g)
+    c = a_t.clone()
+    tiled.adjoint(g, grad_iq=c, accumulate=True)
+    torch.cuda.synchronize()
+    r = H.rel_l2(a_t.cpu().numpy(), a_f.cpu().numpy())
+    say(f"per-tilt tiled vs untiled: rel-L2 {r:.3e}")
+    assert r <= TOL
+    assert H.rel_l2(c.cpu().numpy(), 2 * a_t.cpu().numpy()) <= 1e-6
