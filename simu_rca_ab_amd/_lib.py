@@ -18,10 +18,10 @@ IQ_C64, IQ_F16X2 = 0, 1
 FLAG_COMPOUND, FLAG_ACCUMULATE, FLAG_NOSUM = 1, 2, 4
 ABLAW = {None: 0, "channel": 1, "tx_channel": 2}
 
-# Every symbol include/rca_das.h declares.
+# Every symbol include/rca_das.h and include/rca_svd.h declare.
 EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last_error",
            "rca_version", "rca_set_tuning", "rca_batch_workspace_bytes", "rca_batch_encode",
-           "rca_batch_launch")
+           "rca_batch_launch", "rca_svd_workspace_bytes", "rca_svd_filter")
 
 
 class RcaDasDesc(ctypes.Structure):
@@ -59,7 +59,12 @@ def load():
     lib.rca_adjoint.argtypes = [dp, vp, i, i, vp, i, vp]
     lib.rca_set_tuning.argtypes = [i, i, i, i]
     lib.bfVirtualSourceGeneric_RCA.argtypes = [vp] * 13 + [i, f, f, f, f, f, f, f, f] + [i] * 7 + [vp]
-    for name in ("rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA"):
+    ll, sz = ctypes.c_longlong, ctypes.c_size_t
+    lib.rca_svd_workspace_bytes.argtypes = [ll, i]
+    lib.rca_svd_workspace_bytes.restype = sz
+    lib.rca_svd_filter.argtypes = [vp, vp, ll, i, vp, vp, vp, vp, sz, vp]
+    for name in ("rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA",
+                 "rca_svd_filter"):
         getattr(lib, name).restype = ctypes.c_int
     lib.rca_last_error.restype = ctypes.c_char_p
     lib.rca_version.restype = ctypes.c_char_p

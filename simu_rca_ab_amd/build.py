@@ -11,18 +11,20 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SOURCES = [PKG / "csrc" / "rca_das.hip"]
+SOURCES = [PKG / "csrc" / "rca_das.hip", PKG / "csrc" / "rca_svd.hip"]
 OUT = PKG / "lib" / "librca_das.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", str(ROOT / "include")]
+# the clutter filter calls rocBLAS (ZHERK, ZGEMM) and rocSOLVER (ZHEEVD)
+LIBS = ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def build_library(verbose: bool = False) -> Path:
     OUT.parent.mkdir(parents=True, exist_ok=True)
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [HIPCC, *FLAGS, "-o", str(tmp), *map(str, SOURCES)]
+    cmd = [HIPCC, *FLAGS, "-o", str(tmp), *map(str, SOURCES), *LIBS]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -59,6 +59,16 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(RCA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+}  // namespace
+
+// Shared with the other translation units of the library (rca_svd.hip).
+namespace rca_internal {
+void set_error(const char* msg) { g_err = msg; }
+void clear_error() { g_err.clear(); }
+}  // namespace rca_internal
+
+namespace {
+
 constexpr int kThreads = 256;
 constexpr int kMaxLds = 64 * 1024;
 

@@ -1,0 +1,238 @@
+// SVD clutter filter of a beamformed ensemble on MI355X (gfx950).
+//
+// Reference: Python/clutterfilter/SVDClutterFilter.py:10-23 and
+// MATLAB/lib/clutterFilter/SVDClutterFilter.m:1-45.  The ensemble is the
+// Casorati matrix Y [nPix x nT] (pixels x frames, column-major: the
+// beamformer's [nFrame][nPix] output as is), optionally weighted by
+// 1/sqrt(mask) per pixel (.m:21).  The reference takes the SVD of the Gram
+// matrix Y^H Y (.py:16, .m:23) - Hermitian positive semi-definite, so its
+// singular vectors are its eigenvectors and its singular values its
+// eigenvalues - keeps a band of the vectors (.py:17, .m:26-44) and returns
+// Y V_k V_k^H (.py:18, .m:44).
+//
+// Here everything runs in FP64 on the GPU:
+//   weigh_to_c128   Y = iq / sqrt(mask) as complex128 (hand-written)
+//   ZHERK           G = Y^H Y, upper triangle (rocBLAS, FP64 matrix cores)
+//   ZHEEVD          G = V diag(w) V^H, w ascending (rocSOLVER)
+//   projector       P = sum over kept k of v_k v_k^H (hand-written)
+//   ZGEMM           Y P, in chunks of pixels (rocBLAS)
+//   to_c64          complex64 output in the input layout (hand-written)
+// Projecting with P = V_k V_k^H equals (Y V_k) V_k^H and does not depend on
+// the phase of each eigenvector.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <climits>
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+
+#include "../../include/rca_das.h"
+#include "../../include/rca_svd.h"
+
+namespace rca_internal {
+void set_error(const char* msg);
+void clear_error();
+}  // namespace rca_internal
+
+namespace {
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  rca_internal::set_error(buf);
+  return code;
+}
+
+constexpr long long kChunkPix = 1LL << 20;   // pixels per projection GEMM
+constexpr int kMaxDevices = 64;
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Workspace: Y (complex128 nPix x nT), G / eigenvectors, P (nT x nT),
+// eigenvalues, the eigensolver's off-diagonal scratch and status, the keep
+// flags, and one projected chunk (complex128 chunk x nT).
+struct WsLayout {
+  size_t y, g, p, w, e, info, keep, chunk, total;
+  long long chunk_pix;
+};
+
+WsLayout ws_layout(long long n_pix, int n_t) {
+  WsLayout L;
+  size_t off = 0;
+  L.y = off;
+  off = align256(off + 16 * (size_t)n_pix * n_t);
+  L.g = off;
+  off = align256(off + 16 * (size_t)n_t * n_t);
+  L.p = off;
+  off = align256(off + 16 * (size_t)n_t * n_t);
+  L.w = off;
+  off = align256(off + 8 * (size_t)n_t);
+  L.e = off;
+  off = align256(off + 8 * (size_t)n_t);
+  L.info = off;
+  off = align256(off + sizeof(rocblas_int));
+  L.keep = off;
+  off = align256(off + (size_t)n_t);
+  L.chunk_pix = n_pix < kChunkPix ? n_pix : kChunkPix;
+  L.chunk = off;
+  off = align256(off + 16 * (size_t)L.chunk_pix * n_t);
+  L.total = off;
+  return L;
+}
+
+// y[t][p] = iq[t][p] / sqrt(mask[p]) in FP64 (SVDClutterFilter.m:21-22).
+// Grid: x over pixels, y over frames.
+__global__ void weigh_to_c128(const float2* __restrict__ iq, const float* __restrict__ mask,
+                              long long n_pix, double2* __restrict__ y) {
+  const long long t = blockIdx.y;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n_pix;
+       p += (long long)gridDim.x * blockDim.x) {
+    const float2 v = iq[t * n_pix + p];
+    const double s = mask ? 1.0 / sqrt((double)mask[p]) : 1.0;
+    y[t * n_pix + p] = make_double2((double)v.x * s, (double)v.y * s);
+  }
+}
+
+// P[i + j*n] = sum over kept k of V[i][a] conj(V[j][a]), a = n-1-k: the
+// eigensolver returns ascending eigenvalues, keep[] counts from the largest.
+__global__ void projector(const double2* __restrict__ V, const unsigned char* __restrict__ keep,
+                          int n, double2* __restrict__ P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n) return;
+  double re = 0.0, im = 0.0;
+  for (int k = 0; k < n; ++k) {
+    if (!keep[k]) continue;
+    const int col = n - 1 - k;
+    const double2 a = V[i + (long long)col * n], b = V[j + (long long)col * n];
+    re = fma(a.x, b.x, fma(a.y, b.y, re));
+    im = fma(a.y, b.x, fma(-a.x, b.y, im));
+  }
+  P[i + (long long)j * n] = make_double2(re, im);
+}
+
+__global__ void eig_descending(const double* __restrict__ w, int n, double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = w[n - 1 - k];
+}
+
+// out[t][p0 + p] = c[p + t*ldc] as complex64, p < np.  Grid: x over pixels,
+// y over frames.
+__global__ void to_c64(const double2* __restrict__ c, long long ldc, long long p0, long long np,
+                       long long n_pix, float2* __restrict__ out) {
+  const long long t = blockIdx.y;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < np;
+       p += (long long)gridDim.x * blockDim.x) {
+    const double2 v = c[p + t * ldc];
+    out[t * n_pix + p0 + p] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+std::mutex g_mu;
+rocblas_handle g_handle[kMaxDevices] = {};
+
+int blas_fail(rocblas_status st, const char* what) {
+  return fail(RCA_ERR_HIP, "%s: %s", what, rocblas_status_to_string(st));
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t rca_svd_workspace_bytes(long long n_pix, int n_t) {
+  if (n_pix < 1 || n_t < 1) return 0;
+  return ws_layout(n_pix, n_t).total;
+}
+
+int rca_svd_filter(const void* iq, const float* mask, long long n_pix, int n_t,
+                   const unsigned char* keep, void* out, double* eigval, void* workspace,
+                   size_t workspace_bytes, hipStream_t stream) {
+  rca_internal::clear_error();
+  if (!iq || !out || !keep || !workspace)
+    return fail(RCA_ERR_INVALID, "iq, out, keep and workspace must not be NULL");
+  if (n_pix < 1 || n_t < 1) return fail(RCA_ERR_INVALID, "nPix and nT must be >= 1");
+  if (n_pix > INT_MAX || n_t > 65535)
+    return fail(RCA_ERR_UNSUPPORTED, "nPix <= 2^31-1 and nT <= 65535 are supported");
+  const WsLayout L = ws_layout(n_pix, n_t);
+  if (workspace_bytes < L.total)
+    return fail(RCA_ERR_INVALID, "workspace of %zu bytes, %zu needed", workspace_bytes, L.total);
+  int dev = 0;
+  hipError_t he = hipGetDevice(&dev);
+  if (he != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(RCA_ERR_HIP, "hipGetDevice failed");
+  rocblas_handle h;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_handle[dev]) {
+      const rocblas_status st = rocblas_create_handle(&g_handle[dev]);
+      if (st != rocblas_status_success) return blas_fail(st, "rocblas_create_handle");
+    }
+    h = g_handle[dev];
+  }
+  rocblas_status st = rocblas_set_stream(h, stream);
+  if (st != rocblas_status_success) return blas_fail(st, "rocblas_set_stream");
+
+  unsigned char* ws = static_cast<unsigned char*>(workspace);
+  double2* Y = reinterpret_cast<double2*>(ws + L.y);
+  double2* G = reinterpret_cast<double2*>(ws + L.g);
+  double2* P = reinterpret_cast<double2*>(ws + L.p);
+  double* W = reinterpret_cast<double*>(ws + L.w);
+  double* E = reinterpret_cast<double*>(ws + L.e);
+  rocblas_int* info = reinterpret_cast<rocblas_int*>(ws + L.info);
+  unsigned char* keep_d = ws + L.keep;
+  double2* CH = reinterpret_cast<double2*>(ws + L.chunk);
+  const int ni = (int)n_pix;
+
+  const unsigned gx = (unsigned)((n_pix + 255) / 256 < 1024 ? (n_pix + 255) / 256 : 1024);
+  hipLaunchKernelGGL(weigh_to_c128, dim3(gx, n_t), dim3(256), 0, stream,
+                     static_cast<const float2*>(iq), mask, n_pix, Y);
+  if ((he = hipGetLastError()) != hipSuccess)
+    return fail(RCA_ERR_HIP, "weigh_to_c128: %s", hipGetErrorString(he));
+
+  const double one = 1.0, zero = 0.0;
+  st = rocblas_zherk(h, rocblas_fill_upper, rocblas_operation_conjugate_transpose, n_t, ni, &one,
+                     reinterpret_cast<const rocblas_double_complex*>(Y), ni, &zero,
+                     reinterpret_cast<rocblas_double_complex*>(G), n_t);
+  if (st != rocblas_status_success) return blas_fail(st, "rocblas_zherk");
+  st = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n_t,
+                        reinterpret_cast<rocblas_double_complex*>(G), n_t, W, E, info);
+  if (st != rocblas_status_success) return blas_fail(st, "rocsolver_zheevd");
+
+  if ((he = hipMemcpyAsync(keep_d, keep, (size_t)n_t, hipMemcpyHostToDevice, stream)) != hipSuccess)
+    return fail(RCA_ERR_HIP, "keep upload: %s", hipGetErrorString(he));
+  hipLaunchKernelGGL(projector, dim3((n_t + 63) / 64, n_t), dim3(64), 0, stream, G, keep_d, n_t, P);
+  if ((he = hipGetLastError()) != hipSuccess)
+    return fail(RCA_ERR_HIP, "projector: %s", hipGetErrorString(he));
+
+  const rocblas_double_complex c1(1.0, 0.0), c0(0.0, 0.0);
+  for (long long p0 = 0; p0 < n_pix; p0 += L.chunk_pix) {
+    const long long np = n_pix - p0 < L.chunk_pix ? n_pix - p0 : L.chunk_pix;
+    st = rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, (int)np, n_t, n_t, &c1,
+                       reinterpret_cast<const rocblas_double_complex*>(Y + p0), ni,
+                       reinterpret_cast<const rocblas_double_complex*>(P), n_t, &c0,
+                       reinterpret_cast<rocblas_double_complex*>(CH), (int)np);
+    if (st != rocblas_status_success) return blas_fail(st, "rocblas_zgemm");
+    const unsigned cx = (unsigned)((np + 255) / 256 < 1024 ? (np + 255) / 256 : 1024);
+    hipLaunchKernelGGL(to_c64, dim3(cx, n_t), dim3(256), 0, stream, CH, np, p0, np, n_pix,
+                       static_cast<float2*>(out));
+    if ((he = hipGetLastError()) != hipSuccess)
+      return fail(RCA_ERR_HIP, "to_c64: %s", hipGetErrorString(he));
+  }
+  if (eigval) {
+    hipLaunchKernelGGL(eig_descending, dim3((n_t + 255) / 256), dim3(256), 0, stream, W, n_t, eigval);
+    if ((he = hipGetLastError()) != hipSuccess)
+      return fail(RCA_ERR_HIP, "eig_descending: %s", hipGetErrorString(he));
+  }
+  rocblas_int info_h = 0;
+  if ((he = hipMemcpyAsync(&info_h, info, sizeof(info_h), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+      (he = hipStreamSynchronize(stream)) != hipSuccess)
+    return fail(RCA_ERR_HIP, "status read-back: %s", hipGetErrorString(he));
+  if (info_h != 0) return fail(RCA_ERR_HIP, "rocsolver_zheevd did not converge (info=%d)", (int)info_h);
+  return RCA_OK;
+}
+
+}  // extern "C"

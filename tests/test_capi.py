@@ -1,5 +1,6 @@
-"""The C-ABI library loads and exports every symbol include/rca_das.h declares;
-argument validation returns error codes before any GPU work (CPU-only)."""
+"""The C-ABI library loads and exports every symbol include/rca_das.h and
+include/rca_svd.h declare; argument validation returns error codes before any
+GPU work (CPU-only)."""
 import ctypes
 import re
 from pathlib import Path
@@ -8,12 +9,16 @@ import pytest
 
 from simu_rca_ab_amd import _lib
 
-HEADER = Path(__file__).resolve().parents[1] / "include" / "rca_das.h"
+INCLUDE = Path(__file__).resolve().parents[1] / "include"
+HEADERS = (INCLUDE / "rca_das.h", INCLUDE / "rca_svd.h")
 
 
 def declared_functions():
-    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(\w+)\s*\(", text, flags=re.M)))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
 
 
 def test_header_declares_the_bound_symbols():
@@ -44,3 +49,12 @@ def test_validation_errors_without_gpu():
     _lib.set_tuning(0, 0, 0)
     rc = lib.rca_adjoint(None, None, 1, 1, None, 0, None)
     assert rc == _lib.RCA_ERR_INVALID
+
+
+def test_svd_validation_without_gpu():
+    lib = _lib.load()
+    assert lib.rca_svd_workspace_bytes(0, 4) == 0
+    assert lib.rca_svd_workspace_bytes(1000, 10) >= 16 * 1000 * 10
+    rc = lib.rca_svd_filter(None, None, 1000, 10, None, None, None, None, 0, None)
+    assert rc == _lib.RCA_ERR_INVALID
+    assert b"NULL" in lib.rca_last_error()
