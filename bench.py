@@ -132,6 +132,68 @@ def run_fp16(cfg, dev, n_frame=16, reps=3):
     return res
 
 
+def run_adjoint(dev, reps=3):
+    """DAS forward and adjoint (the backward of the beamformer) on config 4:
+    96+96 elements, 16 tilts, 96x96x192 voxels, 8 frames, Lanczos a=3."""
+    import torch
+    from simu_rca_ab_amd import configs as C, synth
+    cfg = C.get("cfg4")
+    plan = build_plans(cfg, dev)[0]
+    one = synth.frame(cfg, device=dev)[0].to(dev)
+    ph = torch.exp(1j * torch.linspace(0, 2, cfg.n_frame, device=dev)).to(torch.complex64)
+    iq = (one * ph[:, None, None, None]).contiguous()   # [8, nTx, nCh, Ns]
+    g = torch.Generator(device=dev).manual_seed(4004)
+    gout = torch.randn((cfg.n_frame, 1, plan.n_pix), dtype=torch.complex64, device=dev, generator=g)
+    res = {}
+    for name, fn in (("forward", lambda: plan.forward(iq, want_mask=False)),
+                     ("adjoint", lambda: plan.adjoint(gout))):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[name + "_ms"] = dt * 1e3
+        res[name + "_updates_per_s"] = plan.updates(cfg.n_frame) / dt
+    res["workload"] = ("cfg4: 96+96 elements, 16 tilts -10..10 deg, 1024 samples, 96x96x192 voxels, "
+                       "8 frames, Lanczos a=3")
+    return res
+
+
+def run_svd(dev, n_frame=200, reps=3):
+    """SVD clutter filter of a config-2-shaped Doppler ensemble: 128x128x256
+    voxels x 200 frames (synthetic: a rank-3 tissue part 40 dB above random
+    blood), high-pass keeping eigenvectors 3..199."""
+    import torch
+    from simu_rca_ab_amd import clutterfilter as CF
+    npix = 128 * 128 * 256
+    g = torch.Generator(device=dev).manual_seed(2002)
+    t = torch.arange(n_frame, device=dev, dtype=torch.float32)
+    iq = torch.randn((n_frame, npix), dtype=torch.complex64, device=dev, generator=g)
+    for r in range(3):   # tissue: slowly varying, 40 dB above
+        sp = torch.randn(npix, dtype=torch.complex64, device=dev, generator=g) * 100.0
+        iq += torch.exp(2j * torch.pi * 0.01 * (r + 1) * t).to(torch.complex64)[:, None] * sp
+    keep = [False] * 3 + [True] * (n_frame - 3)
+    out, ev = CF.svd_filter(iq, keep)   # warm-up: rocBLAS / rocSOLVER set-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out, ev = CF.svd_filter(iq, keep)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    ev = ev.cpu().numpy()
+    gram_flop = 8.0 * npix * n_frame * n_frame / 2   # Gram, one triangle
+    proj_flop = 8.0 * npix * n_frame * n_frame       # projection ZGEMM
+    return {"ms": dt * 1e3, "pixels": npix, "frames": n_frame,
+            "pixel_frames_per_s": npix * n_frame / dt,
+            "fp64_tflops": (gram_flop + proj_flop) / dt / 1e12,
+            "tissue_to_blood_eigengap": float(ev[2] / ev[3]),
+            "workload": "cfg2 shape: 128x128x256 voxels x 200 frames, complex64 in/out, FP64 "
+                        "Gram (hand-written MFMA) + eigh (rocSOLVER ZHEEVD) + projection "
+                        "(rocBLAS ZGEMM)"}
+
+
 def run_serving(dev, capacity, n_req=32, load=0.7, seed=3000):
     """Serving stream: Poisson arrivals of requests drawn like the serving
     workload (grid N_xy ~ U{64..160}, N_z = 2 N_xy, tilts in {8,16,32,64},
@@ -238,6 +300,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-I/Q leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--no-svd", action="store_true", help="skip the SVD clutter-filter leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (cfg4) leg")
     ap.add_argument("--dbf", action="store_true",
                     help="add the differentiable-beamforming leg (its first run compiles MIOpen "
                          "Conv3d kernels, which can take minutes)")
@@ -281,6 +345,14 @@ def main():
         extra["fp16_iq"] = {"value": r["fp16"], "fp32_same_ensemble": r["fp32"],
                             "unit": "voxel*tilt*element updates/s", "frames": 16,
                             "vs_plain_fp32_single_frame": r["fp16"] / value}
+    if not args.no_adjoint:
+        log("[bench] adjoint leg (config 4)")
+        extra["adjoint"] = run_adjoint(dev)
+        torch.cuda.empty_cache()
+    if not args.no_svd:
+        log("[bench] SVD clutter-filter leg (128x128x256 x 200 frames)")
+        extra["svd_filter"] = run_svd(dev)
+        torch.cuda.empty_cache()
     if args.dbf:
         log("[bench] differentiable beamforming leg")
         extra["dbf"] = run_dbf(dev)

@@ -8,11 +8,12 @@
  * its SVD (the eigen-decomposition of a Hermitian positive semi-definite
  * matrix), and projects Y on a band of the eigenvectors: Y V_k V_k^H.
  *
- * Plain C: device pointers, sizes and scalars.  Unlike rca_das.h the filter
- * calls rocBLAS (ZHERK, ZGEMM) and rocSOLVER (ZHEEVD), which keep a handle
- * and an internal workspace per device; the caller's workspace holds the
- * FP64 copy of the data.  The call returns when the filter is done (it
- * synchronizes the stream to read the eigensolver's status).
+ * Plain C: device pointers, sizes and scalars.  The Gram matrix is a
+ * hand-written FP64 matrix-core kernel; the filter also calls rocSOLVER
+ * (ZHEEVD) and rocBLAS (ZGEMM, the projection), which keep a handle and an
+ * internal workspace per device; the caller's workspace holds the FP64 copy
+ * of the data.  The call returns when the filter is done (it synchronizes the
+ * stream to read the eigensolver's status).
  */
 #ifndef RCA_SVD_H
 #define RCA_SVD_H

@@ -11,8 +11,9 @@
 // Y V_k V_k^H (.py:18, .m:44).
 //
 // Here everything runs in FP64 on the GPU:
-//   weigh_to_c128   Y = iq / sqrt(mask) as complex128 (hand-written)
-//   ZHERK           G = Y^H Y, upper triangle (rocBLAS, FP64 matrix cores)
+//   gram_f64        G = Y^H Y from the complex64 data: split-K FP64 MFMA tiles
+//                   (hand-written), then a deterministic sum of the splits
+//   weigh_to_c128   Y = iq / sqrt(mask) as complex128 for the projection
 //   ZHEEVD          G = V diag(w) V^H, w ascending (rocSOLVER)
 //   projector       P = sum over kept k of v_k v_k^H (hand-written)
 //   ZGEMM           Y P, in chunks of pixels (rocBLAS)
@@ -49,6 +50,9 @@ int fail(int code, const char* fmt, ...) {
 }
 
 constexpr long long kChunkPix = 1LL << 20;   // pixels per projection GEMM
+constexpr int kGT = 32;                      // Gram tile: frames per side
+constexpr int kGK = 64;                      // Gram: pixels per staged slice
+constexpr int kGRow = 2 * kGT + 16;          // LDS row (one pixel): 64 frames + pad
 constexpr int kMaxDevices = 64;
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -57,8 +61,10 @@ size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 // eigenvalues, the eigensolver's off-diagonal scratch and status, the keep
 // flags, and one projected chunk (complex128 chunk x nT).
 struct WsLayout {
-  size_t y, g, p, w, e, info, keep, chunk, total;
+  size_t y, g, p, w, e, info, keep, chunk, part, total;
   long long chunk_pix;
+  int nb, npairs, splits;
+  long long per_split;
 };
 
 WsLayout ws_layout(long long n_pix, int n_t) {
@@ -81,6 +87,17 @@ WsLayout ws_layout(long long n_pix, int n_t) {
   L.chunk_pix = n_pix < kChunkPix ? n_pix : kChunkPix;
   L.chunk = off;
   off = align256(off + 16 * (size_t)L.chunk_pix * n_t);
+  // Gram tiles: kGT x kGT frames per workgroup over the upper triangle of
+  // tile pairs, split over pixel ranges so that about 2048 workgroups run
+  L.nb = (n_t + kGT - 1) / kGT;
+  L.npairs = L.nb * (L.nb + 1) / 2;
+  long long sp = (2048 + L.npairs - 1) / L.npairs;
+  const long long max_sp = (n_pix + kGK - 1) / kGK;
+  if (sp > max_sp) sp = max_sp;
+  L.per_split = ((n_pix + sp - 1) / sp + kGK - 1) / kGK * kGK;
+  L.splits = (int)((n_pix + L.per_split - 1) / L.per_split);
+  L.part = off;
+  off = align256(off + 16 * (size_t)L.splits * L.npairs * kGT * kGT);
   L.total = off;
   return L;
 }
@@ -96,6 +113,98 @@ __global__ void weigh_to_c128(const float2* __restrict__ iq, const float* __rest
     const double s = mask ? 1.0 / sqrt((double)mask[p]) : 1.0;
     y[t * n_pix + p] = make_double2((double)v.x * s, (double)v.y * s);
   }
+}
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Upper-triangular tile pair of index pr (row-major): bi <= bj < nb.
+__host__ __device__ inline void tile_pair(int pr, int nb, int& bi, int& bj) {
+  bi = 0;
+  while (pr >= nb - bi) {
+    pr -= nb - bi;
+    ++bi;
+  }
+  bj = bi + pr;
+}
+
+// One workgroup: the 32 x 32 complex block G[I0.., J0..] of Y^H Y over the
+// pixels [k_begin, k_end) of its split, in FP64 on the matrix cores.
+// Real MFMAs on the complex product conj(a) b:
+//   re += a_r b_r + a_i b_i,   im += a_r b_i - a_i b_r.
+// The mask weight 1/mask_k (= (1/sqrt(mask_k))^2) scales the b operand.
+// Slices of kGK pixels x (32 + 32) frames are staged in LDS as [pixel][frame],
+// so the 16 lanes that read one pixel's frames read 128 contiguous bytes.
+// v_mfma_f64_16x16x4f64 lane layout (checked on MI355X with a probe of known
+// products): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in
+// lane j + 16*(i%4), register i/4.
+__global__ void __launch_bounds__(256) gram_f64(const float2* __restrict__ iq,
+                                                const float* __restrict__ mask, long long n_pix,
+                                                int n_t, int nb, long long per_split,
+                                                double2* __restrict__ partial) {
+  __shared__ float2 tile[kGK][kGRow];
+  __shared__ double wk[kGK];
+  int bi, bj;
+  tile_pair(blockIdx.x, nb, bi, bj);
+  const int I0 = bi * kGT, J0 = bj * kGT;
+  const long long k_begin = (long long)blockIdx.y * per_split;
+  const long long k_end = min(n_pix, k_begin + per_split);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wi = w & 1, wj = w >> 1;   // the wave's 16 x 16 sub-block
+  d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+  for (long long k0 = k_begin; k0 < k_end; k0 += kGK) {
+    __syncthreads();
+    for (int e = tid; e < 2 * kGT * kGK; e += 256) {
+      const int f = e / kGK, q = e % kGK;   // local frame, pixel (coalesced along pixels)
+      const int frame = f < kGT ? I0 + f : J0 + (f - kGT);
+      const long long p = k0 + q;
+      float2 v = make_float2(0.0f, 0.0f);
+      if (frame < n_t && p < k_end) v = iq[(long long)frame * n_pix + p];
+      tile[q][f] = v;
+    }
+    if (tid < kGK) {
+      const long long p = k0 + tid;
+      wk[tid] = p < k_end ? (mask ? 1.0 / (double)mask[p] : 1.0) : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int ks = 0; ks < kGK; ks += 4) {
+      const int kk = ks + (lane >> 4);
+      const float2 a = tile[kk][wi * 16 + (lane & 15)];
+      const float2 b = tile[kk][kGT + wj * 16 + (lane & 15)];
+      const double s = wk[kk];
+      const double ar = a.x, ai = a.y, br = (double)b.x * s, bim = (double)b.y * s;
+      accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, accr, 0, 0, 0);
+      accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bim, accr, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bim, acci, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, br, acci, 0, 0, 0);
+    }
+  }
+  double2* dst = partial + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * (kGT * kGT);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = wi * 16 + (lane >> 4) + 4 * r, col = wj * 16 + (lane & 15);
+    dst[row * kGT + col] = make_double2(accr[r], acci[r]);
+  }
+}
+
+// G (column-major n_t x n_t) = sum of the splits' blocks, both triangles.
+__global__ void gram_reduce(const double2* __restrict__ partial, int npairs, int splits, int n_t,
+                            int nb, double2* __restrict__ G) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)npairs * kGT * kGT) return;
+  const int pr = (int)(e / (kGT * kGT)), el = (int)(e % (kGT * kGT));
+  int bi, bj;
+  tile_pair(pr, nb, bi, bj);
+  const int i = bi * kGT + el / kGT, j = bj * kGT + el % kGT;
+  if (i >= n_t || j >= n_t) return;
+  double re = 0.0, im = 0.0;
+  for (int s = 0; s < splits; ++s) {
+    const double2 v = partial[((long long)s * npairs + pr) * (kGT * kGT) + el];
+    re += v.x;
+    im += v.y;
+  }
+  G[i + (long long)j * n_t] = make_double2(re, im);
+  if (bi != bj) G[j + (long long)i * n_t] = make_double2(re, -im);
 }
 
 // P[i + j*n] = sum over kept k of V[i][a] conj(V[j][a]), a = n-1-k: the
@@ -193,11 +302,16 @@ int rca_svd_filter(const void* iq, const float* mask, long long n_pix, int n_t,
   if ((he = hipGetLastError()) != hipSuccess)
     return fail(RCA_ERR_HIP, "weigh_to_c128: %s", hipGetErrorString(he));
 
-  const double one = 1.0, zero = 0.0;
-  st = rocblas_zherk(h, rocblas_fill_upper, rocblas_operation_conjugate_transpose, n_t, ni, &one,
-                     reinterpret_cast<const rocblas_double_complex*>(Y), ni, &zero,
-                     reinterpret_cast<rocblas_double_complex*>(G), n_t);
-  if (st != rocblas_status_success) return blas_fail(st, "rocblas_zherk");
+  double2* PART = reinterpret_cast<double2*>(ws + L.part);
+  hipLaunchKernelGGL(gram_f64, dim3(L.npairs, L.splits), dim3(256), 0, stream,
+                     static_cast<const float2*>(iq), mask, n_pix, n_t, L.nb, L.per_split, PART);
+  if ((he = hipGetLastError()) != hipSuccess)
+    return fail(RCA_ERR_HIP, "gram_f64: %s", hipGetErrorString(he));
+  const long long nel = (long long)L.npairs * kGT * kGT;
+  hipLaunchKernelGGL(gram_reduce, dim3((unsigned)((nel + 255) / 256)), dim3(256), 0, stream, PART,
+                     L.npairs, L.splits, n_t, L.nb, G);
+  if ((he = hipGetLastError()) != hipSuccess)
+    return fail(RCA_ERR_HIP, "gram_reduce: %s", hipGetErrorString(he));
   st = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n_t,
                         reinterpret_cast<rocblas_double_complex*>(G), n_t, W, E, info);
   if (st != rocblas_status_success) return blas_fail(st, "rocsolver_zheevd");
