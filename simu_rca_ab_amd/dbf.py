@@ -51,21 +51,26 @@ def volume(out: torch.Tensor, nx: int, ny: int, nz: int):
     return v.real, v.imag
 
 
-def train_step(iq, plan, net, target, opt, grid_shape):
+def train_step(iq, plan, net, target, opt, grid_shape, cnn_dtype=torch.bfloat16):
+    """One step: FP32 beamforming (HIP forward, adjoint in the backward), the
+    CNN under autocast to ``cnn_dtype`` (bf16 convolutions run on the matrix
+    cores; None keeps FP32), FP32 loss."""
     nx, ny, nz = grid_shape
     opt.zero_grad(set_to_none=True)
     y = das(iq, plan, True)
     vr, vi = volume(y, nx, ny, nz)
     # scale the beamformed volume to O(1) before the network
     s = 1.0 / (y.detach().abs().amax() + 1e-12)
-    pr, pi = net(vr * s, vi * s)
+    with torch.autocast("cuda", dtype=cnn_dtype or torch.float32, enabled=cnn_dtype is not None):
+        pr, pi = net(vr * s, vi * s)
+    pr, pi = pr.float(), pi.float()
     loss = torch.mean((pr - target[0]) ** 2) + torch.mean((pi - target[1]) ** 2)
     loss.backward()
     opt.step()
     return loss.detach()
 
 
-def time_cfg4(plan, iq, grid_shape, steps=3, ch=32, seed=4004, log=None):
+def time_cfg4(plan, iq, grid_shape, steps=3, ch=32, seed=4004, log=None, cnn_dtype=torch.bfloat16):
     """Times forward+backward training steps and the adjoint (backward of the
     beamformer) alone.  Returns a dict of milliseconds."""
     dev = iq.device
@@ -80,11 +85,11 @@ def time_cfg4(plan, iq, grid_shape, steps=3, ch=32, seed=4004, log=None):
     x = iq.clone().requires_grad_(True)
     say = log or (lambda *a: None)
     say("[dbf] warm-up step (first use compiles the convolution kernels)")
-    losses = [train_step(x, plan, net, target, opt, grid_shape).item()]
+    losses = [train_step(x, plan, net, target, opt, grid_shape, cnn_dtype).item()]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
-        losses.append(train_step(x, plan, net, target, opt, grid_shape).item())
+        losses.append(train_step(x, plan, net, target, opt, grid_shape, cnn_dtype).item())
         say(f"[dbf] step {i + 1}/{steps} loss {losses[-1]:.5g}")
     torch.cuda.synchronize()
     step_ms = (time.perf_counter() - t0) * 1e3 / steps
@@ -102,4 +107,5 @@ def time_cfg4(plan, iq, grid_shape, steps=3, ch=32, seed=4004, log=None):
     torch.cuda.synchronize()
     fwd_ms = (time.perf_counter() - t0) * 1e3 / steps
     return {"train_step_ms": step_ms, "das_forward_ms": fwd_ms, "das_adjoint_ms": adj_ms,
+            "cnn_dtype": str(cnn_dtype).replace("torch.", "") if cnn_dtype else "float32",
             "losses": losses, "updates_per_pass": plan.updates(iq.shape[0])}
