@@ -110,14 +110,27 @@ __device__ __forceinline__ double sqrt_nr(double x) {
   return fma(d, h, g);
 }
 
-// sin(pi x), cos(pi x) for |x| <= 1/4 (Taylor to x^9 / x^10; |error| < 3e-8).
+// sin(pi x), cos(pi x) for |x| <= 1/(2W) (the Lanczos tap identities take
+// x = e/W with |e| <= 1/2).  Taylor series truncated where the next term is
+// below 2e-8 at the interval's end: W = 2 (|pi x| <= pi/4) to x^9 / x^10,
+// W = 3 (<= pi/6) to x^7 / x^8, W = 4 (<= pi/8) to x^7 / x^6.
+template <int W>
 __device__ __forceinline__ void sincospi_small(float x, float& s, float& c) {
   const float t = x * 3.14159265358979f;
   const float t2 = t * t;
-  s = t * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.75573192e-6f, -1.98412698e-4f), 8.33333333e-3f),
-                        -1.66666667e-1f), 1.0f);
-  c = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.75573192e-7f, 2.48015873e-5f),
-                                      -1.38888889e-3f), 4.16666667e-2f), -0.5f), 1.0f);
+  if constexpr (W <= 2) {
+    s = t * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.75573192e-6f, -1.98412698e-4f), 8.33333333e-3f),
+                          -1.66666667e-1f), 1.0f);
+    c = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.75573192e-7f, 2.48015873e-5f),
+                                        -1.38888889e-3f), 4.16666667e-2f), -0.5f), 1.0f);
+  } else if constexpr (W == 3) {
+    s = t * fmaf(t2, fmaf(t2, fmaf(t2, -1.98412698e-4f, 8.33333333e-3f), -1.66666667e-1f), 1.0f);
+    c = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.48015873e-5f, -1.38888889e-3f), 4.16666667e-2f),
+                      -0.5f), 1.0f);
+  } else {
+    s = t * fmaf(t2, fmaf(t2, fmaf(t2, -1.98412698e-4f, 8.33333333e-3f), -1.66666667e-1f), 1.0f);
+    c = fmaf(t2, fmaf(t2, fmaf(t2, -1.38888889e-3f, 4.16666667e-2f), -0.5f), 1.0f);
+  }
 }
 
 // sin/cos of 2*pi*p for p in [-1/2, 1/2], branch-free: sin/cos of the half
@@ -204,12 +217,15 @@ __device__ __forceinline__ void interp_weights(float d, float* q) {
     const bool up = d >= 0.5f;
     const float e = up ? d - 1.0f : d;
     float se, ce;
-    sincospi_small(e * (1.0f / (float)W), se, ce);
+    sincospi_small<W>(e * (1.0f / (float)W), se, ce);
     float S;  // sin(pi e) from the multiple-angle identity
     if constexpr (W == 2) S = 2.0f * se * ce;
     else if constexpr (W == 3) S = se * fmaf(-4.0f * se, se, 3.0f);
     else S = 4.0f * se * ce * fmaf(-2.0f * se, se, 1.0f);
     const float K = S * ((float)W / kPi2);
+    // w_j = (-1)^j K (se cos(pi j/W) - ce sin(pi j/W)) / x^2: the sign and
+    // K go into the two products once, the tap constants into one FMA
+    const float Ks = K * se, Kc = K * ce;
     constexpr int J0 = -LO - 1;
     constexpr int NJ = LO + HI + 2;
     float wj[NJ];
@@ -226,10 +242,10 @@ __device__ __forceinline__ void interp_weights(float d, float* q) {
         wj[i] = fabsf(e) < 1e-3f ? ser : rat;
       } else {
         const float x = e - (float)j;
-        const float cj = cs_cos(W, j), sj = cs_sin(W, j);
-        const float num = sj == 0.0f ? se * cj : cj == 0.0f ? -ce * sj : se * cj - ce * sj;
-        const float sg = (j & 1) ? -K : K;
-        wj[i] = sg * num * __builtin_amdgcn_rcpf(x * x);
+        const float sg = (j & 1) ? -1.0f : 1.0f;
+        const float cj = sg * cs_cos(W, j), sj = -sg * cs_sin(W, j);
+        const float num = sj == 0.0f ? Ks * cj : cj == 0.0f ? Kc * sj : fmaf(Ks, cj, Kc * sj);
+        wj[i] = num * __builtin_amdgcn_rcpf(x * x);
       }
     }
 #pragma unroll
@@ -424,9 +440,28 @@ __device__ __forceinline__ void tx_bounds(const KArgs& a, double l_lo, double l_
 
 // Transmit terms of one (voxel, tilt), cached in LDS for a chunk of tilts.
 struct alignas(16) TxTerm {
-  double bu;   // transmit sample offset fs*((|P-VS| - d0)/c + t0)
+  int bi;      // transmit sample offset fs*((|P-VS| - d0)/c + t0) = bi + bf,
+  float bf;    //   bi = floor, bf in [0, 1)
   float2 et;   // e^{+j 2 pi frac(Bp)}, Bp = fd*((|P-VS| - d0 - 2z)/c + t0)
 };
+
+// x = i + f with i = floor(x) and f = x - i in [0, 1], computed in FP64.
+// Sample positions are summed from such parts in integer + FP32 arithmetic:
+// the fractions are exact to 2^-25 samples each, far below what the weights
+// and the range test resolve.  (Offsets beyond +-2^30 samples are clamped;
+// such updates are out of range anyway.)
+__device__ __forceinline__ void split_pos(double x, int& i, float& f) {
+  const double fl = fmin(fmax(floor(x), -1073741824.0), 1073741824.0);
+  i = (int)fl;
+  f = (float)(x - fl);
+}
+
+// m = floor(u), d = u - m for u = (bi + bf) + (ri + rf) [+ (ai + af)].
+__device__ __forceinline__ void sample_pos(int i, float f, int& m, float& d) {
+  const float fl = floorf(f);
+  m = i + (int)fl;
+  d = f - fl;
+}
 
 // Coordinates of a workgroup's voxels and their bounding box.
 struct TileBox {
@@ -512,7 +547,7 @@ __device__ __forceinline__ void chunk_tx(const KArgs& a, int t0, int nt, bool rx
       float sn, cs;
       sincos2pi((float)(Bp - rint(Bp)), sn, cs);
       TxTerm x;
-      x.bu = fma(a.kfs, tt, a.ut0);
+      split_pos(fma(a.kfs, tt, a.ut0), x.bi, x.bf);
       x.et = make_float2(cs, sn);
       txc[(j * V + v) * kThreads + tid] = x;
       const float fx = (float)(rx ? pl[v] : pt[v]), fy = (float)(rx ? pt[v] : pl[v]);
@@ -607,7 +642,8 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   const S* iq = reinterpret_cast<const S*>(a.iq);
   __shared__ float s_box[4][6];
   __shared__ float s_scl[R];          // FP16 trace scales of the staged rows
-  __shared__ double s_uab[TC];        // TX_CHANNEL law: per-tilt delay (samples)
+  __shared__ int s_uab_i[TC];         // TX_CHANNEL law: per-tilt delay (samples),
+  __shared__ float s_uab_f[TC];       //   integer and fractional parts
   __shared__ float2 s_eab[TC];        // and phasor
 
   const int tid = threadIdx.x;
@@ -708,7 +744,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
               const double pab = a.fd * ab;
               float sn, cs;
               sincos2pi((float)(pab - rint(pab)), sn, cs);
-              s_uab[tid] = a.fs * ab;
+              split_pos(a.fs * ab, s_uab_i[tid], s_uab_f[tid]);
               s_eab[tid] = make_float2(cs, sn);
             }
           }
@@ -728,7 +764,9 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
             }
             const double dz = pz[v] - ezd;
             const double Rr = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));   // .cu:308-309
-            const double ur = fma(a.kfs, Rr, uabc);                         // .cu:310-311
+            int ri;
+            float rf;
+            split_pos(fma(a.kfs, Rr, uabc), ri, rf);                         // .cu:310-311
             const double ph = fma(a.kfd, Rr, pabc);                         // .cu:371-373
             float sn, cs;
             sincos2pi((float)(ph - rint(ph)), sn, cs);
@@ -739,14 +777,15 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
             for (int j = 0; j < nt; ++j, tv += V * kThreads, rows += FB * wcap) {
               if (!((act[v] >> j) & 1u)) continue;
               const TxTerm x = *tv;
-              double u = x.bu + ur;
-              if (ab_tc) u += s_uab[j];
-              const double mf = floor(u);                                       // .cu:319
-              if (!(mf >= (double)LO && mf < (double)(ns - 1 - HI))) continue;  // .cu:323
-              const int base = (int)mf - LO;
+              int m;
+              float d;
+              if (ab_tc) sample_pos(x.bi + ri + s_uab_i[j], x.bf + rf + s_uab_f[j], m, d);
+              else sample_pos(x.bi + ri, x.bf + rf, m, d);                 // .cu:310-319
+              if (!(m >= LO && m < ns - 1 - HI)) continue;                  // .cu:323
+              const int base = m - LO;
               if ((k > 0 && base < cb) || (!last && base >= cb + step)) continue;
               float q[NT];
-              interp_weights<KIND>((float)(u - mf), q);
+              interp_weights<KIND>(d, q);
               float pr = er * x.et.x - ei * x.et.y;
               float pi = er * x.et.y + ei * x.et.x;
               if (ab_tc) {
@@ -932,9 +971,10 @@ __global__ void __launch_bounds__(kThreads) das_fwd_batch(const int* __restrict_
 template <int KIND, int FB, bool PT>
 __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, float* rw,
                                             int* claim, unsigned act, int nt, float er, float ei,
-                                            double ur, int cb, bool last, int step, int k,
+                                            int ri, float rf, int cb, bool last, int step, int k,
                                             int wcap, int nf, const float2* g, int f0, int t0,
-                                            long long p, bool ab_tc, const double* s_uab,
+                                            long long p, bool ab_tc, const int* s_uab_i,
+                                            const float* s_uab_f,
                                             const float2* s_eab) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
   const int ns = a.n_sample;
@@ -943,14 +983,15 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
   for (int j = 0; j < nt; ++j, tv += kThreads, rw += FB * 2 * wcap) {
     if (!((act >> j) & 1u)) continue;
     const TxTerm x = *tv;
-    double u = x.bu + ur;
-    if (ab_tc) u += s_uab[j];
-    const double mf = floor(u);                                       // .cu:319
-    if (!(mf >= (double)LO && mf < (double)(ns - 1 - HI))) continue;  // .cu:323
-    const int base = (int)mf - LO;
+    int m;
+    float d;
+    if (ab_tc) sample_pos(x.bi + ri + s_uab_i[j], x.bf + rf + s_uab_f[j], m, d);
+    else sample_pos(x.bi + ri, x.bf + rf, m, d);                 // .cu:310-319
+    if (!(m >= LO && m < ns - 1 - HI)) continue;                  // .cu:323
+    const int base = m - LO;
     if ((k > 0 && base < cb) || (!last && base >= cb + step)) continue;
     float q[NT];
-    interp_weights<KIND>((float)(u - mf), q);
+    interp_weights<KIND>(d, q);
     float pr = er * x.et.x - ei * x.et.y;
     float pi = er * x.et.y + ei * x.et.x;
     if (ab_tc) {
@@ -1018,7 +1059,8 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
   static_assert(R <= 64 && kThreads % R == 0, "rows must divide the workgroup");
   TxTerm* txc = reinterpret_cast<TxTerm*>(smem_raw);
   __shared__ float s_box[4][6];
-  __shared__ double s_uab[TC];        // TX_CHANNEL law: per-tilt delay (samples)
+  __shared__ int s_uab_i[TC];         // TX_CHANNEL law: per-tilt delay (samples),
+  __shared__ float s_uab_f[TC];       //   integer and fractional parts
   __shared__ float2 s_eab[TC];        // and phasor
 
   const int tid = threadIdx.x;
@@ -1095,7 +1137,7 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
           const double pab = a.fd * ab;
           float sn, cs;
           sincos2pi((float)(pab - rint(pab)), sn, cs);
-          s_uab[tid] = a.fs * ab;
+          split_pos(a.fs * ab, s_uab_i[tid], s_uab_f[tid]);
           s_eab[tid] = make_float2(cs, sn);
         }
         __syncthreads();
@@ -1105,18 +1147,20 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
           const float w = hann((float)(A * iapF[0]));   // .cu:299
           const double dz = pz[0] - ezd;
           const double Rr = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));   // .cu:308-309
-          const double ur = fma(a.kfs, Rr, uabc);                         // .cu:310-311
+          int ri;
+          float rf;
+          split_pos(fma(a.kfs, Rr, uabc), ri, rf);                         // .cu:310-311
           const double ph = fma(a.kfd, Rr, pabc);                         // .cu:371-373
           float sn, cs;
           sincos2pi((float)(ph - rint(ph)), sn, cs);
           const float er = w * cs, ei = w * sn;
           if (gt1)
-            adj_scatter<KIND, FB, false>(a, txc + tid, myrows, claim, act[0], nt, er, ei, ur, cb,
-                                         last, step, k, wcap, nf, g, f0, t0, p, ab_tc, s_uab,
+            adj_scatter<KIND, FB, false>(a, txc + tid, myrows, claim, act[0], nt, er, ei, ri, rf, cb,
+                                         last, step, k, wcap, nf, g, f0, t0, p, ab_tc, s_uab_i, s_uab_f,
                                          s_eab);
           else
-            adj_scatter<KIND, FB, true>(a, txc + tid, myrows, claim, act[0], nt, er, ei, ur, cb,
-                                        last, step, k, wcap, nf, g, f0, t0, p, ab_tc, s_uab,
+            adj_scatter<KIND, FB, true>(a, txc + tid, myrows, claim, act[0], nt, er, ei, ri, rf, cb,
+                                        last, step, k, wcap, nf, g, f0, t0, p, ab_tc, s_uab_i, s_uab_f,
                                         s_eab);
         }
         __syncthreads();
