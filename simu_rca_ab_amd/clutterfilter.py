@@ -94,30 +94,28 @@ def _as_out(ev, like_numpy: bool):
     return ev.cpu().numpy() if like_numpy else ev
 
 
-def SVDClutterFilter(IQ, mask, wcut):
-    """Python/clutterfilter/SVDClutterFilter.py:10-23 (``mask`` is not used,
-    as in the reference).  Returns ``(IQf, s)``."""
-    nt = (IQ.shape if hasattr(IQ, "shape") else np.shape(IQ))[-1]
+def keep_python(wcut, nt: int) -> np.ndarray:
+    """Eigenvectors SVDClutterFilter.py keeps: ``u[:, int(ncut[0]):int(ncut[1])]``,
+    ``ncut = wcut * Nt`` (.py:12, :17), as a mask over descending eigenvalues."""
     ncut = np.asarray(wcut, dtype=np.float64) * nt
     keep = np.zeros(nt, bool)
     keep[int(ncut[0]):int(ncut[1])] = True
-    frames, back = _to_frames(IQ)
-    out, ev = svd_filter(frames, keep)
-    return back(out), _as_out(ev, not isinstance(IQ, torch.Tensor))
+    return keep
 
 
 def _matlab_round(x):
     return np.sign(x) * np.floor(np.abs(x) + 0.5)
 
 
-def SVDClutterFilterMatlab(IQ, Mask, w, type="band"):   # noqa: A002 (the reference's name)
-    """MATLAB/lib/clutterFilter/SVDClutterFilter.m:1-45.  Returns
-    ``(IQf, eig_val)``; IQf stays divided by sqrt(Mask), as in the reference."""
+def keep_matlab(w, nt: int, kind: str = "band") -> np.ndarray:
+    """Eigenvectors SVDClutterFilter.m keeps (:13, :26-44): ``Ncut =
+    round(w*Nt)`` and the 1-based inclusive bands 'low' 1:Ncut(1), 'high'
+    Ncut(1):end, 'stop' [1:Ncut(1), Ncut(2):end], 'band' Ncut(1):Ncut(2).
+    Raises ValueError for w outside [0, 1] (:6-8) and IndexError where MATLAB
+    would index column 0."""
     w = np.atleast_1d(np.asarray(w, dtype=np.float64))
     if w.min() < 0 or w.max() > 1:
         raise ValueError("w must be is the normalized cutoff frequency [0 1]")
-    shape = IQ.shape if hasattr(IQ, "shape") else np.shape(IQ)
-    nt = shape[-1]
     n = _matlab_round(w * nt).astype(int)
 
     def start(i):
@@ -126,18 +124,36 @@ def SVDClutterFilterMatlab(IQ, Mask, w, type="band"):   # noqa: A002 (the refere
         return n[i] - 1
 
     keep = np.zeros(nt, bool)
-    if type == "low":
+    if kind == "low":
         keep[:n[0]] = True
-    elif type == "high":
+    elif kind == "high":
         keep[start(0):] = True
-    elif type == "stop":
+    elif kind == "stop":
         keep[:n[0]] = True
         keep[start(1):] = True
     else:
         keep[start(0):n[1]] = True
+    return keep
+
+
+def SVDClutterFilter(IQ, mask, wcut):
+    """Python/clutterfilter/SVDClutterFilter.py:10-23 (``mask`` is not used,
+    as in the reference).  Returns ``(IQf, s)``."""
+    nt = (IQ.shape if hasattr(IQ, "shape") else np.shape(IQ))[-1]
+    keep = keep_python(wcut, nt)
+    frames, back = _to_frames(IQ)
+    out, ev = svd_filter(frames, keep)
+    return back(out), _as_out(ev, not isinstance(IQ, torch.Tensor))
+
+
+def SVDClutterFilterMatlab(IQ, Mask, w, type="band"):   # noqa: A002 (the reference's name)
+    """MATLAB/lib/clutterFilter/SVDClutterFilter.m:1-45.  Returns
+    ``(IQf, eig_val)``; IQf stays divided by sqrt(Mask), as in the reference."""
+    shape = IQ.shape if hasattr(IQ, "shape") else np.shape(IQ)
+    keep = keep_matlab(w, shape[-1], type)
     frames, back = _to_frames(IQ)
     mask = None
-    if Mask is not None and np.size(Mask if not isinstance(Mask, torch.Tensor) else Mask.numel()):
+    if Mask is not None and (Mask.numel() if isinstance(Mask, torch.Tensor) else np.size(Mask)):
         if isinstance(Mask, torch.Tensor):
             k = Mask.dim()
             mask = Mask.permute(*reversed(range(k))).reshape(-1)
@@ -148,4 +164,5 @@ def SVDClutterFilterMatlab(IQ, Mask, w, type="band"):   # noqa: A002 (the refere
     return back(out), _as_out(ev, not isinstance(IQ, torch.Tensor))
 
 
-__all__ = ["svd_filter", "SVDClutterFilter", "SVDClutterFilterMatlab"]
+__all__ = ["svd_filter", "keep_python", "keep_matlab", "SVDClutterFilter",
+           "SVDClutterFilterMatlab"]
