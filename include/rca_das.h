@@ -51,7 +51,22 @@ enum {
                               (NoSum/bfVirtualSourceGeneric_RCA_Nosum.cu:228-229, 376-380) */
 };
 
-typedef enum { RCA_ABLAW_NONE = 0, RCA_ABLAW_CHANNEL = 1, RCA_ABLAW_TX_CHANNEL = 2 } rca_ablaw;
+/*
+ * Receive aberration laws: delays in seconds added to the receive time of
+ * flight (and so to the sample position and the phase).
+ *   CHANNEL        ab_law_rx [nChannel]          (Hyp1_save/bfVirtualSourceGeneric_RCA_ab_1.cu:306-311)
+ *   TX_CHANNEL     ab_law_rx [nTx][nChannel]     (Hyp2/bfVirtualSourceGeneric_RCA_ab_2.cu:306-311)
+ *   PIXEL_CHANNEL  ab_law_rx and ab_law_tx [nChannel][nPix], either may be
+ *                  NULL (exemple_paul/bfVirtualSourceGenericTxRx.cu:268-275,
+ *                  292-294: AbLawRx[idxPix + idxChannel*nPix] joins the
+ *                  receive delay, AbLawTx[...] the total delay)
+ */
+typedef enum {
+  RCA_ABLAW_NONE = 0,
+  RCA_ABLAW_CHANNEL = 1,
+  RCA_ABLAW_TX_CHANNEL = 2,
+  RCA_ABLAW_PIXEL_CHANNEL = 3
+} rca_ablaw;
 
 /*
  * Beamforming geometry and parameters of one RCA half: the argument list of
@@ -70,7 +85,7 @@ typedef struct {
   const float* vs_z;         /* [nTx] virtualSourceZ */
   const float* min_dist;     /* [nTx] minDistVirtualSource */
   const float* ab_law_rx;    /* receive aberration delays (s) or NULL */
-  int ab_law_mode;           /* rca_ablaw: [nChannel] or [nTx][nChannel] */
+  int ab_law_mode;           /* rca_ablaw: [nChannel], [nTx][nChannel] or [nChannel][nPix] */
   int n_pix;
   int n_channel;
   int n_tx;
@@ -91,6 +106,9 @@ typedef struct {
    * per (tilt, line).  0 = no hint (workgroups take consecutive voxels).  The
    * results do not depend on it. */
   int depth_run;
+  /* RCA_ABLAW_PIXEL_CHANNEL only: the transmit-side law [nChannel][nPix] (s),
+   * or NULL (AbLawTx of bfVirtualSourceGenericTxRx.cu:122, 275, 294). */
+  const float* ab_law_tx;
 } rca_das_desc;
 
 /*

@@ -9,6 +9,8 @@
 //   * variants     NoSum/bfVirtualSourceGeneric_RCA_Nosum.cu:228-229,275-278,376-380
 //                  Hyp1_save/bfVirtualSourceGeneric_RCA_ab_1.cu:306-311 (AbLawRx[ch])
 //                  Hyp2/bfVirtualSourceGeneric_RCA_ab_2.cu:306-311   (AbLawRx[tx*nCh+ch])
+//                  exemple_paul/bfVirtualSourceGenericTxRx.cu:268-275, 292-294
+//                    (AbLawRx/AbLawTx[idxPix + ch*nPix], on the RCA geometry)
 //
 // The arithmetic follows the reference formula term by term, in the precision
 // R of the instantiation: R=double is the FP64 oracle, R=float is the FP32
@@ -104,8 +106,9 @@ template <typename R> inline void weights(const Interp& I, R d, R* q) {
 
 struct Args {
   const float *gx, *gy, *gz, *ex, *ey, *ez, *vx, *vy, *vz, *d0;
-  const float* ablaw;  // may be null
-  int ablaw_mode;      // 0 none, 1 per channel, 2 per (tx, channel)
+  const float* ablaw;     // may be null
+  int ablaw_mode;         // 0 none, 1 per channel, 2 per (tx, channel), 3 per (channel, pixel)
+  const float* ablaw_tx;  // mode 3 transmit-side law, may be null
   int receive_x;
   float apx, apy, apz, fs, fd, c, t0, fnum;
   int nPix, nCh, nTx, nFrame, nSample;
@@ -130,10 +133,12 @@ template <typename R> inline bool mask_transmit(const Args& A, int p, int t) {
 // Per-(pixel, tilt) geometry shared by the forward and the adjoint.
 template <typename R> struct PixTx {
   R px, py, pz, invC, delayTx, apF, phase0;
+  int p;
 };
 
 template <typename R> inline PixTx<R> pix_tx(const Args& A, int p, int t) {
   PixTx<R> G;
+  G.p = p;
   G.px = A.gx[p]; G.py = A.gy[p]; G.pz = A.gz[p];
   G.invC = R(1) / (R)A.c;                      // RCA.cu:164
   G.apF = G.pz / (R)A.fnum;                    // RCA.cu:174
@@ -162,7 +167,13 @@ inline bool channel_term(const Args& A, const PixTx<R>& G, int t, int ch, R* apo
   R delayRx = std::sqrt(aper * aper + dzr * dzr) * G.invC;              // RCA.cu:308-309
   if (A.ablaw_mode == 1) delayRx = (R)A.ablaw[ch] + delayRx;            // Hyp1.cu:306-311
   else if (A.ablaw_mode == 2) delayRx = (R)A.ablaw[t * A.nCh + ch] + delayRx;  // Hyp2.cu:306-311
-  R u = (G.delayTx + delayRx + (R)A.t0) * (R)A.fs;                     // RCA.cu:310-311
+  R abTx = 0;
+  if (A.ablaw_mode == 3) {                                              // TxRx.cu:268-275
+    const long long i = G.p + (long long)ch * A.nPix;
+    if (A.ablaw) delayRx = (R)A.ablaw[i] + delayRx;                     // TxRx.cu:292-293
+    if (A.ablaw_tx) abTx = (R)A.ablaw_tx[i];
+  }
+  R u = (G.delayTx + delayRx + (R)A.t0 + abTx) * (R)A.fs;             // RCA.cu:310-311, TxRx.cu:294
   R mf = std::floor(u);                                                 // RCA.cu:319
   // RCA.cu:323 — bounds kept in floating point so no int overflow can occur
   if (!(mf >= (R)A.I.lo && mf < (R)(A.nSample - 1 - A.I.hi))) return true;
@@ -263,11 +274,13 @@ Args make_args(const float* gx, const float* gy, const float* gz, const float* e
                const float* ez, const float* vx, const float* vy, const float* vz, const float* d0,
                int receive_x, float apx, float apy, float apz, float fs, float fd, float c, float t0,
                float fnum, int nPix, int nCh, int nTx, int nFrame, int nSample, int interp,
-               int lanczos_a, int mask_tx, const float* ablaw, int ablaw_mode) {
+               int lanczos_a, int mask_tx, const float* ablaw, int ablaw_mode,
+               const float* ablaw_tx) {
   Args A;
   A.gx = gx; A.gy = gy; A.gz = gz; A.ex = ex; A.ey = ey; A.ez = ez;
   A.vx = vx; A.vy = vy; A.vz = vz; A.d0 = d0;
-  A.ablaw = ablaw; A.ablaw_mode = ablaw ? ablaw_mode : 0;
+  A.ablaw = ablaw; A.ablaw_tx = ablaw_mode == 3 ? ablaw_tx : nullptr;
+  A.ablaw_mode = (ablaw || A.ablaw_tx) ? ablaw_mode : 0;
   A.receive_x = receive_x;
   A.apx = apx; A.apy = apy; A.apz = apz;
   A.fs = fs; A.fd = fd; A.c = c; A.t0 = t0; A.fnum = fnum;
@@ -293,10 +306,10 @@ extern "C" {
       const float *ez, const float *vx, const float *vy, const float *vz, const float *d0,       \
       int receive_x, float apx, float apy, float apz, float fs, float fd, float c, float t0,     \
       float fnum, int nPix, int nCh, int nTx, int nFrame, int nSample, int interp, int lanczos_a, \
-      int mask_tx, const float *ablaw, int ablaw_mode
+      int mask_tx, const float *ablaw, int ablaw_mode, const float *ablaw_tx
 #define ORACLE_PASS                                                                            \
   gx, gy, gz, ex, ey, ez, vx, vy, vz, d0, receive_x, apx, apy, apz, fs, fd, c, t0, fnum, nPix, \
-      nCh, nTx, nFrame, nSample, interp, lanczos_a, mask_tx, ablaw, ablaw_mode
+      nCh, nTx, nFrame, nSample, interp, lanczos_a, mask_tx, ablaw, ablaw_mode, ablaw_tx
 
 // FP64 oracle of the reference kernel (data given in FP64; FP32 inputs are
 // promoted exactly by the caller).

@@ -47,7 +47,7 @@ def load():
     fp = ctypes.POINTER(ctypes.c_float)
     dp = ctypes.POINTER(ctypes.c_double)
     I, F = ctypes.c_int, ctypes.c_float
-    geo = [fp] * 10 + [I, F, F, F, F, F, F, F, F, I, I, I, I, I, I, I, I, fp, I]
+    geo = [fp] * 10 + [I, F, F, F, F, F, F, F, F, I, I, I, I, I, I, I, I, fp, I, fp]
     lib.oracle_rca_forward_f64.argtypes = [dp, dp, dp] + geo + [I, I]
     lib.oracle_rca_forward_f32.argtypes = [fp, fp, fp] + geo + [I, I]
     lib.oracle_rca_adjoint_f64.argtypes = [dp, dp, I] + geo + [I]
@@ -107,24 +107,26 @@ def prepare(Geometry: dict, Transmit: dict) -> dict:
 
 
 def _geo_args(grid, prep, params, n_frame, n_sample, interp, lanczos_a, mask_tx, ablaw,
-              ablaw_mode):
+              ablaw_mode, ablaw_tx=None):
     fp = ctypes.c_float
     gx, gy, gz = grid
     ablaw_arr = None if ablaw is None else _f32(ablaw)
-    keep = [gx, gy, gz, ablaw_arr]
+    ablaw_tx_arr = None if ablaw_tx is None else _f32(ablaw_tx)
+    keep = [gx, gy, gz, ablaw_arr, ablaw_tx_arr]
     args = [_p(gx, fp), _p(gy, fp), _p(gz, fp)]
     args += [_p(prep[k], fp) for k in ("ex", "ey", "ez", "vx", "vy", "vz", "d0")]
     args += [int(params["receive_x"]), float(prep["ap"][0]), float(prep["ap"][1]),
              float(prep["ap"][2]), float(params["fs"]), float(params["fd"]), float(params["c"]),
              float(params["t0"]), float(params["fnumber"]), gx.size, prep["ex"].size,
              prep["vx"].size, int(n_frame), int(n_sample), int(interp), int(lanczos_a),
-             int(mask_tx), None if ablaw_arr is None else _p(ablaw_arr, fp), int(ablaw_mode)]
+             int(mask_tx), None if ablaw_arr is None else _p(ablaw_arr, fp), int(ablaw_mode),
+             None if ablaw_tx_arr is None else _p(ablaw_tx_arr, fp)]
     return args, keep
 
 
 def kernel_forward(data_il, grid, prep, params, interp=INTERP_QUADRATIC, lanczos_a=3,
                    mask_tx=False, precision=64, nosum=False, ablaw=None, ablaw_mode=0,
-                   nthreads=0):
+                   nthreads=0, ablaw_tx=None):
     """Per-tilt forward of the reference kernel (RCA.cu:103-391).
 
     data_il: interleaved real array [nFrame][nTx][nCh][2*Ns] (any shape with
@@ -143,7 +145,7 @@ def kernel_forward(data_il, grid, prep, params, interp=INTERP_QUADRATIC, lanczos
     out = np.zeros(nout, dt)
     mask = np.zeros(npix * ntx, dt)
     args, keep = _geo_args(grid, prep, params, n_frame, n_sample, interp, lanczos_a, mask_tx,
-                           ablaw, ablaw_mode)
+                           ablaw, ablaw_mode, ablaw_tx)
     fn = lib.oracle_rca_forward_f64 if precision == 64 else lib.oracle_rca_forward_f32
     rc = fn(_p(out, ct), _p(mask, ct), _p(data, ct), *args, int(nosum), int(nthreads))
     del keep
@@ -155,7 +157,7 @@ def kernel_forward(data_il, grid, prep, params, interp=INTERP_QUADRATIC, lanczos
 
 
 def kernel_adjoint(g, grid, prep, params, n_frame, interp=INTERP_QUADRATIC, lanczos_a=3,
-                   mask_tx=False, ablaw=None, ablaw_mode=0, nthreads=0):
+                   mask_tx=False, ablaw=None, ablaw_mode=0, nthreads=0, ablaw_tx=None):
     """FP64 adjoint of the per-tilt forward.  g: complex [nFrame, 1|nTx, nPix].
     Returns grad complex [nFrame, nTx, nCh, Ns]."""
     lib = load()
@@ -166,7 +168,7 @@ def kernel_adjoint(g, grid, prep, params, n_frame, interp=INTERP_QUADRATIC, lanc
     grad = np.zeros(2 * n_frame * ntx * nch * ns, np.float64)
     gr = g.view(np.float64).ravel()
     args, keep = _geo_args(grid, prep, params, n_frame, ns, interp, lanczos_a, mask_tx, ablaw,
-                           ablaw_mode)
+                           ablaw_mode, ablaw_tx)
     dp = ctypes.c_double
     rc = lib.oracle_rca_adjoint_f64(_p(grad, dp), _p(gr, dp), int(gtx), *args, int(nthreads))
     del keep

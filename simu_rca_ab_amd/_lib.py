@@ -16,7 +16,7 @@ RCA_OK, RCA_ERR_INVALID, RCA_ERR_UNSUPPORTED, RCA_ERR_HIP, RCA_ERR_BUDGET = rang
 INTERP = {"quadratic": 0, "lanczos": 1, "lanczos3": 2, "lanczos4": 3}
 IQ_C64, IQ_F16X2 = 0, 1
 FLAG_COMPOUND, FLAG_ACCUMULATE, FLAG_NOSUM = 1, 2, 4
-ABLAW = {None: 0, "channel": 1, "tx_channel": 2}
+ABLAW = {None: 0, "channel": 1, "tx_channel": 2, "pixel_channel": 3}
 
 # Every symbol include/rca_das.h and include/rca_svd.h declare.
 EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last_error",
@@ -33,7 +33,7 @@ class RcaDasDesc(ctypes.Structure):
         "interp")] + [(n, ctypes.c_float) for n in (
         "aperture_x", "aperture_y", "aperture_z", "sampling_frequency",
         "demodulation_frequency", "speed_of_sound", "start_time", "fnumber")] + [
-        ("depth_run", ctypes.c_int)]
+        ("depth_run", ctypes.c_int), ("ab_law_tx", ctypes.c_void_p)]
 
 
 class RcaError(RuntimeError):

@@ -23,7 +23,8 @@ from ..plan import RcaPlan, interpolation_name, prepare_geometry
 
 __all__ = ["bfVirtualSourceGeneric_RCA", "bfVirtualSourceGeneric_RCA_Nosum",
            "bfVirtualSourceGeneric_RCA_ab_1", "bfVirtualSourceGeneric_RCA_ab_2",
-           "bfVirtualSourceGeneric_RCA_Nosum_ab", "RcaInputError", "RcaWarning"]
+           "bfVirtualSourceGeneric_RCA_Nosum_ab", "bfVirtualSourceGeneric_RCA_TxRx",
+           "RcaInputError", "RcaWarning"]
 
 _ID = "bfVirtualSourceGeneric_RCA"
 
@@ -276,6 +277,16 @@ def bfVirtualSourceGeneric_RCA_Nosum_ab(*args, device=None):
     return _run(args, device, nosum=True, ablaw="channel")
 
 
+def bfVirtualSourceGeneric_RCA_TxRx(*args, device=None):
+    """Per-(voxel, line) aberration laws on the RCA geometry: ``Param.AbLawRx``
+    joins the receive delay and ``Param.AbLawTx`` the total delay of every
+    update, both indexed AbLaw(:)[idxPix + idxChannel*nPix], i.e. MATLAB arrays
+    [nPix, nChannel] or [s1, s2, s3, nChannel] (the laws of
+    exemple_paul/bfVirtualSourceGenericTxRx.cu:268-275, 292-294, .m:612-613).
+    An empty law counts as zero."""
+    return _run(args, device, nosum=False, ablaw="pixel_channel")
+
+
 def _run(args, device, nosum, ablaw):
     if len(args) == 0:
         from .. import build
@@ -288,6 +299,8 @@ def _run(args, device, nosum, ablaw):
     Transmit, Param, Options = _validate(data, Geometry, Transmit, Param, Options)
     if ablaw is not None and "AbLawRx" not in Param:
         raise RcaInputError("AbLawRxMissing", "Receive aberration law ('Param.AbLawRx') is required!")
+    if ablaw == "pixel_channel" and "AbLawTx" not in Param:
+        raise RcaInputError("AbLawTxMissing", "Transmit aberration law ('Param.AbLawTx') is required!")
 
     as_torch = isinstance(data, torch.Tensor)
     if device is None:
@@ -315,9 +328,13 @@ def _run(args, device, nosum, ablaw):
     Par = {k: _scalar(Param[k]) for k in ("samplingFrequency", "demodulationFrequency",
                                           "speedOfSound", "startTime", "fnumber")}
     if ablaw is not None:
-        law = Param["AbLawRx"]
-        law = (law.detach().cpu().numpy() if isinstance(law, torch.Tensor) else np.asarray(law))
-        Par["AbLawRx"] = law.astype(np.float32).ravel(order="F")   # single(Param.AbLawRx)
+        for key in ("AbLawRx", "AbLawTx") if ablaw == "pixel_channel" else ("AbLawRx",):
+            law = Param[key]
+            law = (law.detach().cpu().numpy() if isinstance(law, torch.Tensor) else np.asarray(law))
+            if law.size:                       # single(Param.AbLaw..), column-major
+                Par[key] = law.astype(np.float32).ravel(order="F")
+        if "AbLawRx" not in Par and "AbLawTx" not in Par:
+            ablaw = None                       # both laws empty: no delay at all
         Par["AbLawMode"] = ablaw
     plan = RcaPlan.from_reference(Geo, dict(Transmit), Par, Options, n_sample=ns_iq, device=device,
                                   depth_run=grid_shape[0])

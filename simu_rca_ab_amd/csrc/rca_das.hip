@@ -76,6 +76,7 @@ constexpr int kMaxLds = 64 * 1024;
 // the FP64 scalars are derived on the host from the FP32 parameters.
 struct KArgs {
   const float *gx, *gy, *gz, *ex, *ey, *ez, *vx, *vy, *vz, *d0, *ab;
+  const float* abt;    // PIXEL_CHANNEL transmit-side law (ab is the receive side)
   const void* iq;
   const float* scale;
   float2* out;
@@ -283,65 +284,6 @@ __device__ __forceinline__ bool tx_mask(const KArgs& a, float vx, float vy, floa
 // rotation weights by the caller).
 __device__ __forceinline__ float2 ld_sample(const float2* s, int i) { return s[i]; }
 __device__ __forceinline__ float2 ld_sample(const __half2* s, int i) { return __half22float2(s[i]); }
-
-// Geometry of one (voxel, tilt, line) update shared by forward and adjoint.
-struct Upd {
-  float w;   // Hann weight (0 outside the aperture)
-  int m;     // floor(u)
-  float d;   // u - m
-  float p;   // demodulation phase in cycles, reduced to [-1/2, 1/2]
-  bool in_ap, in_rng;
-};
-
-template <int LO, int HI>
-__device__ __forceinline__ Upd update_geom(const KArgs& a, double px, double py, double pz,
-                                           double apF, double iapF, double Bu, double Bp,
-                                           double exd, double eyd, double ezd, double uab,
-                                           double pab) {
-  Upd r;
-  const double A = a.receive_x ? fabs(px - exd) : fabs(py - eyd);   // .cu:284-291
-  r.in_ap = A < apF;                                                // .cu:294
-  r.in_rng = false;
-  r.w = 0.0f;
-  r.m = 0;
-  r.d = 0.0f;
-  r.p = 0.0f;
-  if (!r.in_ap) return r;
-  r.w = hann((float)(A * iapF));                                     // .cu:299
-  const double dz = pz - ezd;
-  const double R = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));       // .cu:308-309
-  const double u = fma(a.kfs, R, Bu + uab);                          // .cu:310-311
-  const double mf = floor(u);                                        // .cu:319
-  r.in_rng = mf >= (double)LO && mf < (double)(a.n_sample - 1 - HI); // .cu:323
-  if (!r.in_rng) return r;
-  const double ph = fma(a.kfd, R, Bp + pab);                         // .cu:371-373
-  r.m = (int)mf;
-  r.d = (float)(u - mf);
-  r.p = (float)(ph - rint(ph));
-  return r;
-}
-
-// Per-(voxel, tilt) transmit terms: Bu = fs*((|P-VS| - d0)/c + t0),
-// Bp = fd*((|P-VS| - d0 - 2 z)/c + t0).
-__device__ __forceinline__ void tx_terms(const KArgs& a, double px, double py, double pz,
-                                         double vx, double vy, double vz, double d0, double& Bu,
-                                         double& Bp) {
-  const double dl = a.receive_x ? (py - vy) : (px - vx);             // .cu:256-269
-  const double dz = pz - vz;
-  const double tt = sqrt_nr(fma(dl, dl, dz * dz)) - d0;
-  Bu = fma(a.kfs, tt, a.ut0);
-  Bp = fma(a.kfd, fma(-2.0, pz, tt), a.pt0);
-}
-
-__device__ __forceinline__ void ab_terms(const KArgs& a, int t, int c, double& uab, double& pab) {
-  uab = 0.0;
-  pab = 0.0;
-  if (a.ab_mode) {
-    const double ab = a.ab[a.ab_mode == RCA_ABLAW_CHANNEL ? c : t * a.n_ch + c];
-    uab = a.fs * ab;
-    pab = a.fd * ab;
-  }
-}
 
 // ----------------------------------------------------------------------------
 // Voxel tiles.
@@ -581,18 +523,61 @@ __device__ __forceinline__ void chunk_tx_bounds(const KArgs& a, const TileBox& b
 // rounding, s0 a multiple of E.  Returns false when the line's aperture
 // misses every voxel (min |lateral| >= max z/F): such a line contributes
 // nothing.
+// PIXEL_CHANNEL law of voxel p on line c in seconds: the receive-side delay
+// joins the receive time of flight and the transmit-side one the total delay
+// (bfVirtualSourceGenericTxRx.cu:268-275, 292-294); either array may be absent.
+__device__ __forceinline__ double pixel_law(const KArgs& a, int c, long long p) {
+  const long long i = (long long)c * a.n_pix + p;
+  double l = 0.0;
+  if (a.ab) l = (double)a.ab[i];
+  if (a.abt) l += (double)a.abt[i];
+  return l;
+}
+
+// Range, in samples, of the PIXEL_CHANNEL law over the workgroup's voxels
+// that a tilt of the chunk insonifies (lo > hi when there is none).  The four
+// waves meet in buf[slot]; callers alternate the slot between consecutive
+// lines, so a wave that reaches the next line early never overwrites what a
+// slower wave still reads.
+template <int V>
+__device__ __forceinline__ void pixel_law_range(const KArgs& a, const double (&law)[V],
+                                                const unsigned (&act)[V], const bool (&valid)[V],
+                                                float2 (*buf)[4], int slot, int tid, float& lo,
+                                                float& hi) {
+  float l = INFINITY, h = -INFINITY;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    if (valid[v] && act[v]) {
+      const float x = (float)(a.fs * law[v]);
+      l = fminf(l, x);
+      h = fmaxf(h, x);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    l = fminf(l, __shfl_xor(l, o, 64));
+    h = fmaxf(h, __shfl_xor(h, o, 64));
+  }
+  if ((tid & 63) == 0) buf[slot][tid >> 6] = make_float2(l, h);
+  __syncthreads();
+  const float2 b0 = buf[slot][0], b1 = buf[slot][1], b2 = buf[slot][2], b3 = buf[slot][3];
+  lo = uni(fminf(fminf(b0.x, b1.x), fminf(b2.x, b3.x)));
+  hi = uni(fmaxf(fmaxf(b0.y, b1.y), fmaxf(b2.y, b3.y)));
+}
+
 template <int LO, int HI, int E>
 __device__ __forceinline__ bool line_window(const KArgs& a, const TileBox& b, double e_lat,
                                             double ezd, double uabc, float bu_lo, float bu_hi,
                                             bool ab_tc, int t0, int nt, int c, int lane, int& s0,
-                                            int& whi) {
+                                            int& whi, float px_lo = 0.0f, float px_hi = 0.0f) {
   double A0, A1, Z0, Z1;
   abs_range((double)b.lat_lo - e_lat, (double)b.lat_hi - e_lat, A0, A1);
   abs_range((double)b.zlo - ezd, (double)b.zhi - ezd, Z0, Z1);
   if (!(A0 < b.apmax)) return false;
   A1 = fmin(A1, b.apmax);
-  float r_lo = (float)(a.kfs * sqrt(A0 * A0 + Z0 * Z0) + uabc);
-  float r_hi = (float)(a.kfs * sqrt(A1 * A1 + Z1 * Z1) + uabc);
+  if (!(px_lo <= px_hi)) return false;   // PIXEL_CHANNEL law: no active voxel
+  float r_lo = (float)(a.kfs * sqrt(A0 * A0 + Z0 * Z0) + uabc) + px_lo;
+  float r_hi = (float)(a.kfs * sqrt(A1 * A1 + Z1 * Z1) + uabc) + px_hi;
   if (ab_tc) {   // per-tilt delays of the TX_CHANNEL law: their range over the chunk
     float lo = INFINITY, hi = -INFINITY;
     if (lane < nt) lo = hi = (float)(a.fs * a.ab[(long long)(t0 + lane) * a.n_ch + c]);
@@ -628,8 +613,9 @@ __device__ __forceinline__ bool line_window(const KArgs& a, const TileBox& b, do
 //
 // CMP: compound over tilts (acc per voxel and frame).  Otherwise TC = 1 and
 // accT holds the current tilt's sums over the lines, or one line's terms with
-// nosum.
-template <int KIND, int V, int FB, int TC, bool CMP, typename S>
+// nosum.  PX: the PIXEL_CHANNEL law (a separate instantiation, so the other
+// kernels keep their register budget).
+template <int KIND, int V, int FB, int TC, bool CMP, typename S, bool PX = false>
 __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned by,
                                           unsigned char* smem_raw) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
@@ -645,6 +631,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   __shared__ int s_uab_i[TC];         // TX_CHANNEL law: per-tilt delay (samples),
   __shared__ float s_uab_f[TC];       //   integer and fractional parts
   __shared__ float2 s_eab[TC];        // and phasor
+  __shared__ float2 s_pxr[2][4];      // PIXEL_CHANNEL law: per-wave ranges
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -660,12 +647,14 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   const long long tstride = (long long)a.n_ch * ns;
   const bool rx = a.receive_x != 0;
   const bool ab_tc = a.ab_mode == RCA_ABLAW_TX_CHANNEL;
+  constexpr bool ab_px = PX;
   bool ok;   // scratch for voxel_of at the stores
 
   // receive-axis and transmit-axis coordinates, depth, aperture half-width
   double pl[V], pt[V], pz[V], apF[V], iapF[V];
   bool valid[V];
   const TileBox box = tile_voxels<V, kTileCols, 4>(a, bx, tid, rx, pl, pt, pz, apF, iapF, valid, s_box);
+  int line_no = 0;   // lines visited, alternates the s_pxr slot
 
   float2 acc[V][FB];
   float macc[V];
@@ -707,9 +696,22 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
 #pragma unroll
           for (int fi = 0; fi < FB; ++fi) accT[v][fi] = make_float2(0.0f, 0.0f);
       }
+      double lawv[V];   // PIXEL_CHANNEL law of each voxel on this line (s)
+#pragma unroll
+      for (int v = 0; v < V; ++v) lawv[v] = 0.0;
+      float px_lo = 0.0f, px_hi = 0.0f;
+      if constexpr (ab_px) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const long long p = voxel_of<kTileCols, 4>(a, bx, V, v, tid, ok);
+          if (valid[v]) lawv[v] = pixel_law(a, c, p);
+        }
+        pixel_law_range<V>(a, lawv, act, valid, s_pxr, line_no & 1, tid, px_lo, px_hi);
+        ++line_no;
+      }
       int s0, whi;
       if (line_window<LO, HI, E>(a, box, e_lat, ezd, uabc, bu_lo, bu_hi, ab_tc, t0, nt, c, lane, s0,
-                                 whi)) {
+                                 whi, px_lo, px_hi)) {
         const int npass = whi - s0 > wcap ? 1 + (whi - s0 - wcap + step - 1) / step : 1;
 
         for (int k = 0; k < npass; ++k) {
@@ -764,10 +766,12 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
             }
             const double dz = pz[v] - ezd;
             const double Rr = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));   // .cu:308-309
+            const double uab = ab_px ? a.fs * lawv[v] : uabc;
+            const double pab = ab_px ? a.fd * lawv[v] : pabc;
             int ri;
             float rf;
-            split_pos(fma(a.kfs, Rr, uabc), ri, rf);                         // .cu:310-311
-            const double ph = fma(a.kfd, Rr, pabc);                         // .cu:371-373
+            split_pos(fma(a.kfs, Rr, uab), ri, rf);                          // .cu:310-311
+            const double ph = fma(a.kfd, Rr, pab);                          // .cu:371-373
             float sn, cs;
             sincos2pi((float)(ph - rint(ph)), sn, cs);
             const float er = w * cs, ei = w * sn;
@@ -897,10 +901,11 @@ __host__ __device__ constexpr int tc_max(int fb, int sbytes) {
   return r > RCA_TC_MAX ? RCA_TC_MAX : (r < 1 ? 1 : r);
 }
 
-template <int KIND, int V, int FB, bool CMP, typename S>
+template <int KIND, int V, int FB, bool CMP, typename S, bool PX>
 __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  fwd_block<KIND, V, FB, CMP ? tc_max(FB, sizeof(S)) : 1, CMP, S>(a, blockIdx.x, blockIdx.y, smem_raw);
+  fwd_block<KIND, V, FB, CMP ? tc_max(FB, sizeof(S)) : 1, CMP, S, PX>(a, blockIdx.x, blockIdx.y,
+                                                                     smem_raw);
 }
 
 // ----------------------------------------------------------------------------
@@ -1050,7 +1055,7 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
   }
 }
 
-template <int KIND, int FB, int TC>
+template <int KIND, int FB, int TC, bool PX>
 __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned by,
                                           unsigned char* smem_raw) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
@@ -1062,6 +1067,7 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
   __shared__ int s_uab_i[TC];         // TX_CHANNEL law: per-tilt delay (samples),
   __shared__ float s_uab_f[TC];       //   integer and fractional parts
   __shared__ float2 s_eab[TC];        // and phasor
+  __shared__ float2 s_pxr[2][4];      // PIXEL_CHANNEL law: per-wave ranges
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -1078,7 +1084,9 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
   const int step = max(wcap - NT + 1, 1);   // pass advance
   const bool rx = a.receive_x != 0;
   const bool ab_tc = a.ab_mode == RCA_ABLAW_TX_CHANNEL;
+  constexpr bool ab_px = PX;   // PIXEL_CHANNEL law: its own instantiation
   const bool gt1 = a.gout_tx == 1;
+  int line_no = 0;   // lines visited, alternates the s_pxr slot
 
   double pl[1], pt[1], pz[1], apF[1], iapF[1];
   bool valid[1];
@@ -1110,9 +1118,16 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
         uabc = a.fs * ab;
         pabc = a.fd * ab;
       }
+      double lawv[1] = {0.0};   // PIXEL_CHANNEL law of the voxel on this line (s)
+      float px_lo = 0.0f, px_hi = 0.0f;
+      if constexpr (ab_px) {
+        if (valid[0]) lawv[0] = pixel_law(a, c, p);
+        pixel_law_range<1>(a, lawv, act, valid, s_pxr, line_no & 1, tid, px_lo, px_hi);
+        ++line_no;
+      }
       int s0, whi;
       if (!line_window<LO, HI, 1>(a, box, e_lat, ezd, uabc, bu_lo, bu_hi, ab_tc, t0, nt, c, lane,
-                                  s0, whi))
+                                  s0, whi, px_lo, px_hi))
         continue;
       const int npass = whi - s0 > wcap ? 1 + (whi - s0 - wcap + step - 1) / step : 1;
 
@@ -1147,10 +1162,12 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
           const float w = hann((float)(A * iapF[0]));   // .cu:299
           const double dz = pz[0] - ezd;
           const double Rr = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));   // .cu:308-309
+          const double uab = ab_px ? a.fs * lawv[0] : uabc;
+          const double pab = ab_px ? a.fd * lawv[0] : pabc;
           int ri;
           float rf;
-          split_pos(fma(a.kfs, Rr, uabc), ri, rf);                         // .cu:310-311
-          const double ph = fma(a.kfd, Rr, pabc);                         // .cu:371-373
+          split_pos(fma(a.kfs, Rr, uab), ri, rf);                          // .cu:310-311
+          const double ph = fma(a.kfd, Rr, pab);                          // .cu:371-373
           float sn, cs;
           sincos2pi((float)(ph - rint(ph)), sn, cs);
           const float er = w * cs, ei = w * sn;
@@ -1184,10 +1201,10 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
   __syncthreads();
 }
 
-template <int KIND, int FB>
+template <int KIND, int FB, bool PX>
 __global__ void __launch_bounds__(kThreads) das_adj(const KArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  adj_block<KIND, FB, tc_max(FB, 8)>(a, blockIdx.x, blockIdx.y, smem_raw);
+  adj_block<KIND, FB, tc_max(FB, 8), PX>(a, blockIdx.x, blockIdx.y, smem_raw);
 }
 
 __global__ void fill_zero(float* __restrict__ p, long long n) {
@@ -1212,9 +1229,14 @@ int check_desc(const rca_das_desc* d) {
     return fail(RCA_ERR_INVALID, "virtual-source pointers are NULL");
   if (d->interp < RCA_INTERP_QUADRATIC || d->interp > RCA_INTERP_LANCZOS4)
     return fail(RCA_ERR_INVALID, "unknown interpolation %d", d->interp);
-  if (d->ab_law_mode < 0 || d->ab_law_mode > 2)
+  if (d->ab_law_mode < RCA_ABLAW_NONE || d->ab_law_mode > RCA_ABLAW_PIXEL_CHANNEL)
     return fail(RCA_ERR_INVALID, "unknown aberration-law mode %d", d->ab_law_mode);
-  if (d->ab_law_mode && !d->ab_law_rx) return fail(RCA_ERR_INVALID, "ab_law_rx is NULL");
+  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) {
+    if (!d->ab_law_rx && !d->ab_law_tx)
+      return fail(RCA_ERR_INVALID, "ab_law_rx and ab_law_tx are both NULL");
+  } else if (d->ab_law_mode && !d->ab_law_rx) {
+    return fail(RCA_ERR_INVALID, "ab_law_rx is NULL");
+  }
   if (!(d->speed_of_sound > 0.0f) || !(d->sampling_frequency > 0.0f))
     return fail(RCA_ERR_INVALID, "speed of sound and sampling frequency must be > 0");
   if (!(d->fnumber != 0.0f)) return fail(RCA_ERR_INVALID, "fnumber must be non-zero");
@@ -1227,7 +1249,9 @@ KArgs make_args(const rca_das_desc* d, int n_frame) {
   a.gx = d->grid_x; a.gy = d->grid_y; a.gz = d->grid_z;
   a.ex = d->ele_x; a.ey = d->ele_y; a.ez = d->ele_z;
   a.vx = d->vs_x; a.vy = d->vs_y; a.vz = d->vs_z; a.d0 = d->min_dist;
-  a.ab = d->ab_law_rx; a.ab_mode = d->ab_law_rx ? d->ab_law_mode : 0;
+  a.ab = d->ab_law_rx;
+  a.abt = d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL ? d->ab_law_tx : nullptr;
+  a.ab_mode = d->ab_law_mode;
   const double fs = d->sampling_frequency, fd = d->demodulation_frequency, c = d->speed_of_sound;
   a.kfs = fs / c; a.kfd = fd / c;
   a.ut0 = fs * (double)d->start_time; a.pt0 = fd * (double)d->start_time;
@@ -1267,7 +1291,7 @@ constexpr size_t kFwdLds = 48 * 1024, kFwdLds1 = 32 * 1024;
 
 template <typename S> struct SName;
 
-template <int KIND, int V, int FB, bool CMP, typename S>
+template <int KIND, int V, int FB, bool CMP, typename S, bool PX = false>
 int launch_fwd_t(KArgs a, hipStream_t st) {
   const FwdLayout L = fwd_layout(a, V, FB, sizeof(S), CMP ? tc_max(FB, sizeof(S)) : 1,
                                  g_tune_lds ? (size_t)g_tune_lds * 1024 : FB == 1 ? kFwdLds1 : kFwdLds);
@@ -1278,15 +1302,24 @@ int launch_fwd_t(KArgs a, hipStream_t st) {
   const unsigned gx = (unsigned)fwd_blocks(a.n_pix, a.depth_run, V);
   const unsigned gy = (unsigned)((a.n_frame + FB - 1) / FB);
   if (gx == 0 || gy == 0) return RCA_OK;
-  hipLaunchKernelGGL((das_fwd<KIND, V, FB, CMP, S>), dim3(gx, gy, 1), dim3(kThreads), L.lds, st, a);
+  hipLaunchKernelGGL((das_fwd<KIND, V, FB, CMP, S, PX>), dim3(gx, gy, 1), dim3(kThreads), L.lds, st, a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "das_fwd launch");
 }
 
 // Instantiated shapes: compounding (V, FB) in (1|2, 1), (1, 2|4|8); per-tilt
-// and per-line outputs (1, 1|4).
+// and per-line outputs (1, 1|4); with the PIXEL_CHANNEL law (1, 1|4).
 template <int KIND, typename S>
 int launch_fwd_k(const KArgs& a, int V, int FB, hipStream_t st) {
+  if (a.ab_mode == RCA_ABLAW_PIXEL_CHANNEL) {
+    if (V != 1 || (FB != 1 && FB != 4))
+      return fail(RCA_ERR_UNSUPPORTED, "no pixel-law kernel for V=%d FB=%d", V, FB);
+    if (a.compound)
+      return FB == 1 ? launch_fwd_t<KIND, 1, 1, true, S, true>(a, st)
+                     : launch_fwd_t<KIND, 1, 4, true, S, true>(a, st);
+    return FB == 1 ? launch_fwd_t<KIND, 1, 1, false, S, true>(a, st)
+                   : launch_fwd_t<KIND, 1, 4, false, S, true>(a, st);
+  }
   if (a.compound) {
 #define RCA_FWD_CASE(VV, FF) \
     if (V == VV && FB == FF) return launch_fwd_t<KIND, VV, FF, true, S>(a, st);
@@ -1348,7 +1381,7 @@ AdjLayout adj_layout(const KArgs& a, int fb, int tc_kernel, size_t budget) {
   return L;
 }
 
-template <int KIND, int FB>
+template <int KIND, int FB, bool PX = false>
 int launch_adj_t(KArgs a, hipStream_t st) {
   const AdjLayout L = adj_layout(a, FB, tc_max(FB, 8), g_tune_lds ? (size_t)g_tune_lds * 1024 : kAdjLds);
   if (L.lds > (size_t)kMaxLds || L.wcap < Interp<KIND>::LO + Interp<KIND>::HI + 1)
@@ -1357,13 +1390,17 @@ int launch_adj_t(KArgs a, hipStream_t st) {
   a.wcap = L.wcap;
   const dim3 grid((unsigned)tile_blocks<1, 1>(a.n_pix, a.depth_run, 1),
                   (unsigned)((a.n_frame + FB - 1) / FB));
-  hipLaunchKernelGGL((das_adj<KIND, FB>), grid, dim3(kThreads), L.lds, st, a);
+  hipLaunchKernelGGL((das_adj<KIND, FB, PX>), grid, dim3(kThreads), L.lds, st, a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "das_adj launch");
 }
 
 template <int KIND>
 int launch_adj_k(const KArgs& a, int FB, hipStream_t st) {
+  if (a.ab_mode == RCA_ABLAW_PIXEL_CHANNEL) {   // instantiated for FB = 1 only
+    if (FB != 1) return fail(RCA_ERR_UNSUPPORTED, "no pixel-law adjoint kernel for FB=%d", FB);
+    return launch_adj_t<KIND, 1, true>(a, st);
+  }
   switch (FB) {
     case 1: return launch_adj_t<KIND, 1>(a, st);
     case 2: return launch_adj_t<KIND, 2>(a, st);
@@ -1419,8 +1456,6 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
   if (!trace_fits(1, 1))
     return fail(RCA_ERR_UNSUPPORTED, "nSample=%d exceeds the LDS trace capacity (%d bytes)",
                 d->n_sample, kMaxLds);
-  if (d->ab_law_mode == RCA_ABLAW_TX_CHANNEL && !d->ab_law_rx)
-    return fail(RCA_ERR_INVALID, "ab_law_rx is NULL");
   KArgs a = make_args(d, n_frame);
   a.iq = iq;
   a.scale = trace_scale;
@@ -1438,6 +1473,10 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
   if (!a.compound && fb > 1) fb = fb >= 4 ? 4 : 1;
   while (fb > 1 && !trace_fits(fb, v)) fb >>= 1;    // a tuned FB that does not fit
   if (!trace_fits(fb, v)) v = 1;
+  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) {   // the instantiated pixel-law shapes
+    v = 1;
+    fb = fb >= 4 ? 4 : 1;
+  }
   if (iq_dtype == RCA_IQ_C64) return launch_fwd_s<float2>(a, d->interp, v, fb, stream);
   return launch_fwd_s<__half2>(a, d->interp, v, fb, stream);
 }
@@ -1461,6 +1500,7 @@ int rca_adjoint(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_
   // one frame per workgroup measured fastest (more tilts per chunk fit LDS)
   int fb = g_tune_fb ? g_tune_fb : 1;
   while (fb > 1 && fb > n_frame) fb >>= 1;
+  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) fb = 1;   // the instantiated pixel-law shape
   // every workgroup adds its windows to the traces: start from zero
   if (!a.accumulate) {
     const long long n = 2LL * n_frame * d->n_tx * d->n_channel * d->n_sample;
@@ -1559,6 +1599,8 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
       const rca_das_desc* d = E.half[h];
       int rc = check_desc(d);
       if (rc) return rc;
+      if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL)
+        return fail(RCA_ERR_UNSUPPORTED, "entry %d: per-(pixel, channel) laws are not batched", q);
       const int dr = d->depth_run >= 64 ? d->depth_run : 0;
       if (n_pix >= 0 && (d->n_pix != n_pix || dr != depth_run))
         return fail(RCA_ERR_INVALID, "entry %d: halves differ in nPix or depth_run", q);

@@ -72,6 +72,24 @@ def _depth_run(g) -> int:
     return int(shape[0]) if len(shape) >= 2 else 0
 
 
+def _law_mode(rx, tx, n_ch, n_tx, n_pix):
+    """Aberration-law layout from its size: [nChannel] per line, [nTx][nChannel]
+    per (tilt, line), [nChannel][nPix] per (line, voxel).  A transmit-side law
+    only exists per (line, voxel)."""
+    if tx is not None:
+        return "pixel_channel"
+    n = rx.numel()
+    fits = [m for m, k in (("channel", n_ch), ("tx_channel", n_tx * n_ch),
+                           ("pixel_channel", n_ch * n_pix)) if n == k]
+    if not fits:
+        raise ValueError(f"aberration law of {n} delays matches no layout (nChannel={n_ch}, "
+                         f"nTx={n_tx}, nPix={n_pix})")
+    if len(fits) > 1:
+        raise ValueError(f"aberration law of {n} delays is ambiguous ({', '.join(fits)}): "
+                         "give its mode")
+    return fits[0]
+
+
 def _dev_f32(a, device):
     if isinstance(a, torch.Tensor):
         t = a.detach().to(device=device, dtype=torch.float32)
@@ -109,7 +127,7 @@ class RcaPlan:
 
     def __init__(self, *, grid, ele, vs, min_dist, receive_x, aperture, fs, fd, c, t0, fnumber,
                  n_sample, interp="quadratic", mask_transmit=False, ab_law=None,
-                 ab_law_mode=None, depth_run=0, device=None):
+                 ab_law_mode=None, depth_run=0, device=None, ab_law_tx=None):
         self.lib = _lib.load()
         self.device = torch.device(device) if device is not None else torch.device(
             "cuda", torch.cuda.current_device())
@@ -131,16 +149,21 @@ class RcaPlan:
             raise ValueError("virtual sources and minDist must have one entry per tilt")
         if interp not in _lib.INTERP:
             raise ValueError(f"unknown interpolation {interp!r}")
-        self.ab_law = None
+        self.ab_law = self.ab_law_tx = None
         mode = 0
-        if ab_law is not None:
-            self.ab_law = _dev_f32(ab_law, dev)
-            if ab_law_mode is None:
-                ab_law_mode = "channel" if self.ab_law.numel() == n_ch else "tx_channel"
-            mode = _lib.ABLAW[ab_law_mode]
-            need = n_ch if mode == 1 else n_tx * n_ch
-            if self.ab_law.numel() != need:
-                raise ValueError(f"aberration law needs {need} delays, got {self.ab_law.numel()}")
+        if ab_law is not None or ab_law_tx is not None:
+            self.ab_law = _dev_f32(ab_law, dev) if ab_law is not None else None
+            self.ab_law_tx = _dev_f32(ab_law_tx, dev) if ab_law_tx is not None else None
+            mode = _lib.ABLAW[ab_law_mode if ab_law_mode is not None else
+                              _law_mode(self.ab_law, self.ab_law_tx, n_ch, n_tx, n_pix)]
+            need = {1: n_ch, 2: n_tx * n_ch, 3: n_ch * n_pix}[mode]
+            for name, law in (("AbLawRx", self.ab_law), ("AbLawTx", self.ab_law_tx)):
+                if law is not None and law.numel() != need:
+                    raise ValueError(f"{name} needs {need} delays, got {law.numel()}")
+            if mode != 3 and self.ab_law_tx is not None:
+                raise ValueError("a transmit-side law (AbLawTx) needs the per-(pixel, channel) mode")
+            if mode != 3 and self.ab_law is None:
+                raise ValueError("this aberration-law mode needs AbLawRx")
         self.n_pix, self.n_channel, self.n_tx, self.n_sample = n_pix, n_ch, n_tx, int(n_sample)
         self.interp = interp
         self.receive_x = bool(receive_x)
@@ -150,6 +173,7 @@ class RcaPlan:
         d.vs_x, d.vs_y, d.vs_z = (v.data_ptr() for v in self.vs)
         d.min_dist = self.min_dist.data_ptr()
         d.ab_law_rx = self.ab_law.data_ptr() if self.ab_law is not None else None
+        d.ab_law_tx = self.ab_law_tx.data_ptr() if self.ab_law_tx is not None else None
         d.ab_law_mode = mode
         d.n_pix, d.n_channel, d.n_tx, d.n_sample = n_pix, n_ch, n_tx, self.n_sample
         d.receive_x = int(self.receive_x)
@@ -187,6 +211,7 @@ class RcaPlan:
                    interp=interpolation_name(Options),
                    mask_transmit=bool(Options.get("flagMaskTransmit", True)),
                    ab_law=Param.get("AbLawRx"), ab_law_mode=Param.get("AbLawMode"),
+                   ab_law_tx=Param.get("AbLawTx"),
                    depth_run=_depth_run(Geometry["gridBfZ"]) if depth_run is None else depth_run,
                    device=device)
 

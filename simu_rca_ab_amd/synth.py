@@ -39,12 +39,15 @@ def scatterers(cfg: C.RcaConfig):
     return pos, amp
 
 
-def echoes(cfg: C.RcaConfig, half: C.Half, pos, amp, device="cpu", out=None, geom=None):
+def echoes(cfg: C.RcaConfig, half: C.Half, pos, amp, device="cpu", out=None, geom=None,
+           extra_delay=None):
     """Analytic I/Q of one frame of one half: complex64 [nTilt, nCh, Ns].
 
     ``geom`` optionally replaces the FP64 transmit/receive geometry with
     given arrays (keys vx, vy, vz, d0 per tilt and e per element), e.g. the
-    FP32-rounded values a beamformer receives."""
+    FP32-rounded values a beamformer receives.  ``extra_delay`` (seconds,
+    [nCh] or [nTilt, nCh]) delays every echo on that element, as an
+    aberrating layer in front of the probe would."""
     dev = torch.device(device)
     f64 = torch.float64
     if geom is None:
@@ -67,12 +70,17 @@ def echoes(cfg: C.RcaConfig, half: C.Half, pos, amp, device="cpu", out=None, geo
     else:
         rx = torch.sqrt((ys[None, :] - e[:, None]) ** 2 + zs[None, :] ** 2)
     chan = torch.arange(nc, device=dev)[:, None, None] * ns
+    if extra_delay is not None:
+        xd = torch.as_tensor(np.asarray(extra_delay, dtype=np.float64), device=dev)
+        xd = xd.reshape(1, nc).expand(nt, nc) if xd.numel() == nc else xd.reshape(nt, nc)
     for t in range(nt):
         if half.receive_x:
             tx = torch.sqrt((ys - vy[t]) ** 2 + (zs - vz[t]) ** 2) - d0[t]          # [S]
         else:
             tx = torch.sqrt((xs - vx[t]) ** 2 + (zs - vz[t]) ** 2) - d0[t]
         tau = (tx[None, :] + rx) / cfg.c                                             # [C, S]
+        if extra_delay is not None:
+            tau = tau + xd[t][:, None]
         ncen = torch.floor((tau + cfg.t0) * cfg.fs).to(torch.int64)
         n = ncen[..., None] + win                                                    # [C, S, W]
         tn = n.to(f64) / cfg.fs - cfg.t0

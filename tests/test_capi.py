@@ -58,3 +58,49 @@ def test_svd_validation_without_gpu():
     rc = lib.rca_svd_filter(None, None, 1000, 10, None, None, None, None, 0, None)
     assert rc == _lib.RCA_ERR_INVALID
     assert b"NULL" in lib.rca_last_error()
+
+
+def header_struct_fields(name):
+    text = re.sub(r"/\*.*?\*/", "", (INCLUDE / "rca_das.h").read_text(), flags=re.S)
+    body = re.search(r"typedef struct \{(.*?)\}\s*" + name + ";", text, flags=re.S).group(1)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"(const\s+)?(\w+)\s*(\*?)\s*(.*)", decl)
+        kind = "ptr" if m.group(3) else m.group(2)
+        for n in m.group(4).split(","):
+            fields.append((n.strip().lstrip("*").strip(), kind))
+    return fields
+
+
+def test_desc_mirror_matches_header():
+    kinds = {ctypes.c_void_p: "ptr", ctypes.c_int: "int", ctypes.c_float: "float"}
+    mirror = [(n, kinds[t]) for n, t in _lib.RcaDasDesc._fields_]
+    assert mirror == header_struct_fields("rca_das_desc")
+
+
+def _desc_with_dummy_pointers(mode):
+    d = _lib.RcaDasDesc()
+    for n, t in _lib.RcaDasDesc._fields_:
+        if t is ctypes.c_void_p and not n.startswith("ab_law"):
+            setattr(d, n, 256)   # never dereferenced: validation fails first
+    d.n_pix, d.n_channel, d.n_tx, d.n_sample = 10, 4, 2, 64
+    d.speed_of_sound, d.sampling_frequency, d.fnumber = 1540.0, 20e6, 1.0
+    d.ab_law_mode = mode
+    return d
+
+
+@pytest.mark.parametrize("mode,msg", [(1, b"ab_law_rx is NULL"), (2, b"ab_law_rx is NULL"),
+                                      (3, b"both NULL"), (4, b"unknown aberration-law mode"),
+                                      (-1, b"unknown aberration-law mode")])
+def test_ablaw_validation_without_gpu(mode, msg):
+    lib = _lib.load()
+    d = _desc_with_dummy_pointers(mode)
+    rc = lib.rca_forward(ctypes.byref(d), None, 0, None, 1, None, None, 0, None)
+    assert rc == _lib.RCA_ERR_INVALID
+    assert msg in lib.rca_last_error()
+    rc = lib.rca_adjoint(ctypes.byref(d), None, 1, 1, None, 0, None)
+    assert rc == _lib.RCA_ERR_INVALID
+    assert msg in lib.rca_last_error()

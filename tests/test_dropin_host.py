@@ -107,3 +107,34 @@ def test_product_prep_matches_oracle_prep(half_index):
         np.testing.assert_array_equal(a["vs"][i], b[k])
     np.testing.assert_array_equal(a["d0"], b["d0"])
     np.testing.assert_array_equal(np.float32(a["aperture"]), b["ap"])
+
+
+@pytest.mark.parametrize("fn,drop,ident", [
+    ("bfVirtualSourceGeneric_RCA_ab_1", "AbLawRx", "AbLawRxMissing"),
+    ("bfVirtualSourceGeneric_RCA_ab_2", "AbLawRx", "AbLawRxMissing"),
+    ("bfVirtualSourceGeneric_RCA_Nosum_ab", "AbLawRx", "AbLawRxMissing"),
+    ("bfVirtualSourceGeneric_RCA_TxRx", "AbLawRx", "AbLawRxMissing"),
+    ("bfVirtualSourceGeneric_RCA_TxRx", "AbLawTx", "AbLawTxMissing"),
+])
+def test_missing_aberration_law(fn, drop, ident):
+    import simu_rca_ab_amd.beamformer as B
+    d, G, T, P, Op = _args()
+    P = {**P, "AbLawRx": np.zeros(64, np.float32), "AbLawTx": np.zeros(64, np.float32)}
+    P.pop(drop)
+    with pytest.raises(RcaInputError) as e:
+        getattr(B, fn)(d, G, T, P, Op)
+    assert e.value.identifier == f"bfVirtualSourceGeneric_RCA:{ident}"
+
+
+def test_law_layout_from_size():
+    from simu_rca_ab_amd.plan import _law_mode
+    n_ch, n_tx, n_pix = 64, 11, 300
+    z = lambda n: torch.zeros(n)   # noqa: E731
+    assert _law_mode(z(n_ch), None, n_ch, n_tx, n_pix) == "channel"
+    assert _law_mode(z(n_ch * n_tx), None, n_ch, n_tx, n_pix) == "tx_channel"
+    assert _law_mode(z(n_ch * n_pix), None, n_ch, n_tx, n_pix) == "pixel_channel"
+    assert _law_mode(None, z(n_ch * n_pix), n_ch, n_tx, n_pix) == "pixel_channel"
+    with pytest.raises(ValueError, match="matches no layout"):
+        _law_mode(z(n_ch + 1), None, n_ch, n_tx, n_pix)
+    with pytest.raises(ValueError, match="ambiguous"):
+        _law_mode(z(n_ch * n_tx), None, n_ch, n_tx, n_tx)
