@@ -194,6 +194,75 @@ def run_svd(dev, n_frame=200, reps=3):
                         "(rocBLAS ZGEMM)"}
 
 
+def sim_case(n_elem=128, n_sub=128, n_tilt=21, n_scat=500, n_sample=1024, fc=7.5e6, seed=4242):
+    """SIMUS-RCA workload of one half of the dataset generator
+    (save_simus_sh.m:44-110, 193-225): 128 row elements transmit and 128
+    column elements receive, each element a line of 128 sub-elements across
+    the aperture, 21 plane-wave tilts -10..10 deg, 350-500 scatterers per
+    event (here 500), 1024 samples in BS100BW mode (fs = 4 fc), flat
+    elevation, 0.5 dB/cm/MHz."""
+    import numpy as np
+    c = 1540.0
+    pitch = 0.2e-3
+    xe = (np.arange(n_elem) - (n_elem - 1) / 2) * pitch
+    ys = (np.arange(n_sub) - (n_sub - 1) / 2) * pitch * n_elem / n_sub
+    t_ir = np.arange(33) / 250e6
+    ir = np.sin(2 * np.pi * fc * t_ir) * np.hanning(t_ir.size)
+
+    def trans(row):
+        pos = np.zeros((n_elem, 5, n_sub), np.float32)
+        for u in range(n_sub):
+            pos[:, 0 if row else 1, u] = xe
+            pos[:, 1 if row else 0, u] = ys[u]
+        return {"frequency": fc, "ElementPos": pos, "ElementWidth": 0.18e-3,
+                "ElementHeight": n_elem * pitch, "ElementElevationFocus": 10000.0, "baffle": 0,
+                "lensCorrection": 0.0, "impulseResponse": ir}
+
+    tp = np.arange(int(2 / fc * 250e6)) / 250e6
+    pulse = np.sin(2 * np.pi * fc * tp) * np.hanning(tp.size)
+    th = np.deg2rad(np.linspace(-10, 10, n_tilt))
+    TXSI = [{"pulseShape": pulse, "Delay": xe * np.sin(t) / c - (xe * np.sin(t) / c).min(),
+             "Apod": np.ones(n_elem)} for t in th]
+    lam = c / fc
+    RXSI = [{"samplingFrequency": 4 * fc, "demodFrequency": fc, "sampleMode": "BS100BW",
+             "startTime": 2 * 10 * lam / c, "numelSample": n_sample, "Apod": np.ones(n_elem),
+             "InputFilter": np.array([0.1, 0.25, 0.3, 0.25, 0.1])} for _ in th]
+    rng = np.random.default_rng(seed)
+    span = n_elem * pitch
+    pos = np.stack([rng.uniform(-span / 3, span / 3, n_scat), rng.uniform(-span / 3, span / 3, n_scat),
+                    rng.uniform(25 * lam, 85 * lam, n_scat), rng.standard_normal(n_scat)], axis=1)
+    pos = np.repeat(pos[:, :, None], n_tilt, axis=2).astype(np.float32)   # one frame
+    Media = {"speedOfSound": c, "attenuation": 0.5, "scattererPos": pos}
+    return Media, trans(True), trans(False), TXSI, RXSI
+
+
+def run_sim(dev, reps=2):
+    """One frame of the SIMUS-RCA generator (sim_case): transmit field, fused
+    receive + sum over scatterers, inverse FFT and decimation.  Throughput in
+    field terms (frequency x scatterer x element x sub-element evaluations,
+    transmit plus receive) per second."""
+    import torch
+    from simu_rca_ab_amd.simulator import RcaSimulator
+    Media, TT, TR, TXSI, RXSI = sim_case()
+    sim = RcaSimulator(Media, TT, TR, TXSI, RXSI, device=dev)
+    pos = torch.from_numpy(Media["scattererPos"].transpose(2, 1, 0).copy()).to(dev)
+    raw = torch.empty((sim.n_tx, sim.n_ele_rx, 2 * sim.n_out), dtype=torch.float32, device=dev)
+    sim.frame(pos, raw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.frame(pos, raw)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    nf, n_scat = sim.P["nf"], pos.shape[2]
+    terms = sim.n_tx * nf * n_scat * sim.n_sub * (sim.n_ele_tx + sim.n_ele_rx)
+    return {"ms_per_frame": dt * 1e3, "field_terms_per_s": terms / dt, "terms_per_frame": terms,
+            "frequencies": nf, "nfft": sim.P["nfft"], "scatterers": int(n_scat),
+            "workload": "SIMUS RCA half-frame: 128 row (tx) + 128 column (rx) elements x 128 "
+                        "sub-elements, 21 tilts, 500 scatterers, 1024 samples BS100BW at 7.5 MHz, "
+                        "flat elevation"}
+
+
 def run_serving(dev, capacity, n_req=32, load=0.7, seed=3000):
     """Serving stream: Poisson arrivals of requests drawn like the serving
     workload (grid N_xy ~ U{64..160}, N_z = 2 N_xy, tilts in {8,16,32,64},
@@ -302,6 +371,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream leg")
     ap.add_argument("--no-svd", action="store_true", help="skip the SVD clutter-filter leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (cfg4) leg")
+    ap.add_argument("--no-sim", action="store_true", help="skip the SIMUS-RCA simulator leg")
     ap.add_argument("--dbf", action="store_true",
                     help="add the differentiable-beamforming leg (its first run compiles MIOpen "
                          "Conv3d kernels, which can take minutes)")
@@ -352,6 +422,10 @@ def main():
     if not args.no_svd:
         log("[bench] SVD clutter-filter leg (128x128x256 x 200 frames)")
         extra["svd_filter"] = run_svd(dev)
+        torch.cuda.empty_cache()
+    if not args.no_sim:
+        log("[bench] SIMUS-RCA simulator leg (one half-frame)")
+        extra["simulator"] = run_sim(dev)
         torch.cuda.empty_cache()
     if args.dbf:
         log("[bench] differentiable beamforming leg")

@@ -18,10 +18,11 @@ IQ_C64, IQ_F16X2 = 0, 1
 FLAG_COMPOUND, FLAG_ACCUMULATE, FLAG_NOSUM = 1, 2, 4
 ABLAW = {None: 0, "channel": 1, "tx_channel": 2, "pixel_channel": 3}
 
-# Every symbol include/rca_das.h and include/rca_svd.h declare.
+# Every symbol include/rca_das.h, include/rca_svd.h and include/rca_sim.h declare.
 EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last_error",
            "rca_version", "rca_set_tuning", "rca_batch_workspace_bytes", "rca_batch_encode",
-           "rca_batch_launch", "rca_svd_workspace_bytes", "rca_svd_filter")
+           "rca_batch_launch", "rca_svd_workspace_bytes", "rca_svd_filter", "rca_sim_txfield",
+           "rca_sim_rxfield", "rca_sim_receive", "rca_sim_spectra_to_raw")
 
 
 class RcaDasDesc(ctypes.Structure):
@@ -34,6 +35,16 @@ class RcaDasDesc(ctypes.Structure):
         "aperture_x", "aperture_y", "aperture_z", "sampling_frequency",
         "demodulation_frequency", "speed_of_sound", "start_time", "fnumber")] + [
         ("depth_run", ctypes.c_int), ("ab_law_tx", ctypes.c_void_p)]
+
+
+class RcaSimReceiveArgs(ctypes.Structure):
+    """Mirror of ``rca_sim_receive_args``."""
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "tx", "r_scat", "r_ele", "k", "ir", "ipfilt", "agl", "apod", "ab_delay", "ab_apod")] + [
+        (n, ctypes.c_float) for n in (
+            "width", "height", "rfocus", "baffle", "lens", "speed_of_sound")] + [
+        (n, ctypes.c_int) for n in ("n_scat", "n_ele", "n_f", "n_sub", "n_tx", "nfft")] + [
+        ("spec", ctypes.c_void_p), ("spec_ab", ctypes.c_void_p)]
 
 
 class RcaError(RuntimeError):
@@ -63,8 +74,13 @@ def load():
     lib.rca_svd_workspace_bytes.argtypes = [ll, i]
     lib.rca_svd_workspace_bytes.restype = sz
     lib.rca_svd_filter.argtypes = [vp, vp, ll, i, vp, vp, vp, vp, sz, vp]
+    lib.rca_sim_txfield.argtypes = [vp] * 8 + [f] * 6 + [i] * 5 + [vp, vp]
+    lib.rca_sim_rxfield.argtypes = [vp] * 5 + [f] * 6 + [i] * 4 + [vp]
+    lib.rca_sim_receive.argtypes = [ctypes.POINTER(RcaSimReceiveArgs), vp]
+    lib.rca_sim_spectra_to_raw.argtypes = [vp, i, i, i, i, vp, vp]
     for name in ("rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA",
-                 "rca_svd_filter"):
+                 "rca_svd_filter", "rca_sim_txfield", "rca_sim_rxfield", "rca_sim_receive",
+                 "rca_sim_spectra_to_raw"):
         getattr(lib, name).restype = ctypes.c_int
     lib.rca_last_error.restype = ctypes.c_char_p
     lib.rca_version.restype = ctypes.c_char_p

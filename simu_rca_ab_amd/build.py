@@ -11,14 +11,15 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SOURCES = [PKG / "csrc" / "rca_das.hip", PKG / "csrc" / "rca_svd.hip"]
+SOURCES = [PKG / "csrc" / "rca_das.hip", PKG / "csrc" / "rca_svd.hip", PKG / "csrc" / "rca_sim.hip"]
 OUT = PKG / "lib" / "librca_das.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", str(ROOT / "include")]
-# the clutter filter calls rocBLAS (ZHERK, ZGEMM) and rocSOLVER (ZHEEVD)
-LIBS = ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-Wl,-rpath,/opt/rocm/lib"]
+# the clutter filter calls rocBLAS (ZGEMM) and rocSOLVER (ZHEEVD); the
+# simulator's inverse transform is hipFFT
+LIBS = ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def build_library(verbose: bool = False) -> Path:

@@ -1,5 +1,5 @@
-"""The C-ABI library loads and exports every symbol include/rca_das.h and
-include/rca_svd.h declare; argument validation returns error codes before any
+"""The C-ABI library loads and exports every symbol include/rca_das.h,
+include/rca_svd.h and include/rca_sim.h declare; argument validation returns error codes before any
 GPU work (CPU-only)."""
 import ctypes
 import re
@@ -10,7 +10,7 @@ import pytest
 from simu_rca_ab_amd import _lib
 
 INCLUDE = Path(__file__).resolve().parents[1] / "include"
-HEADERS = (INCLUDE / "rca_das.h", INCLUDE / "rca_svd.h")
+HEADERS = (INCLUDE / "rca_das.h", INCLUDE / "rca_svd.h", INCLUDE / "rca_sim.h")
 
 
 def declared_functions():
@@ -60,8 +60,8 @@ def test_svd_validation_without_gpu():
     assert b"NULL" in lib.rca_last_error()
 
 
-def header_struct_fields(name):
-    text = re.sub(r"/\*.*?\*/", "", (INCLUDE / "rca_das.h").read_text(), flags=re.S)
+def header_struct_fields(name, header="rca_das.h"):
+    text = re.sub(r"/\*.*?\*/", "", (INCLUDE / header).read_text(), flags=re.S)
     body = re.search(r"typedef struct \{(.*?)\}\s*" + name + ";", text, flags=re.S).group(1)
     fields = []
     for decl in body.split(";"):
@@ -75,10 +75,23 @@ def header_struct_fields(name):
     return fields
 
 
-def test_desc_mirror_matches_header():
+@pytest.mark.parametrize("cls,name,header", [(_lib.RcaDasDesc, "rca_das_desc", "rca_das.h"),
+                                             (_lib.RcaSimReceiveArgs, "rca_sim_receive_args",
+                                              "rca_sim.h")])
+def test_struct_mirrors_match_headers(cls, name, header):
     kinds = {ctypes.c_void_p: "ptr", ctypes.c_int: "int", ctypes.c_float: "float"}
-    mirror = [(n, kinds[t]) for n, t in _lib.RcaDasDesc._fields_]
-    assert mirror == header_struct_fields("rca_das_desc")
+    mirror = [(n, kinds[t]) for n, t in cls._fields_]
+    assert mirror == header_struct_fields(name, header)
+
+
+def test_sim_validation_without_gpu():
+    lib = _lib.load()
+    rc = lib.rca_sim_txfield(*([None] * 8), 1.0, 1.0, 1e4, 0.0, 0.0, 1540.0, 1, 1, 1, 1, 1, None, None)
+    assert rc == _lib.RCA_ERR_INVALID and b"NULL" in lib.rca_last_error()
+    rc = lib.rca_sim_receive(None, None)
+    assert rc == _lib.RCA_ERR_INVALID
+    rc = lib.rca_sim_spectra_to_raw(None, 1, 64, 32, 4, None, None)
+    assert rc == _lib.RCA_ERR_INVALID
 
 
 def _desc_with_dummy_pointers(mode):
