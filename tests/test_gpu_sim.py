@@ -136,3 +136,61 @@ def test_frame_api_matches_dropin(dev):
     torch.cuda.synchronize()
     want = SimUS_CUDARCA(M, TT, TR, TX, RX, device=dev)
     np.testing.assert_array_equal(raw.cpu().numpy().transpose(2, 1, 0), want)
+
+
+def test_simulated_point_focuses_through_beamformer(dev):
+    """The simulator's Raw feeds the beamformer drop-in as real data
+    (IQ = RF(2n) - j RF(2n+1)): a point scatterer simulated on the GPU and
+    beamformed on the GPU focuses at its voxel, and the GPU chain equals the
+    oracle chain (simulator oracle -> beamformer oracle)."""
+    from oracle import rca_oracle as O
+    from simu_rca_ab_amd.beamformer import bfVirtualSourceGeneric_RCA
+    say("case simulator -> beamformer: one point scatterer, 32+32 elements, 7 tilts +-15 deg")
+    n_ele, n_tx, fc, c = 32, 7, 7.5e6, 1540.0
+    M, TT, TR, TX, RX = rca_case(n_ele=n_ele, n_sub=8, n_tx=n_tx, n_scat=1, n_sample=512, fc=fc,
+                                 attenuation=0.0)
+    lam = c / fc
+    xe = (np.arange(n_ele) - (n_ele - 1) / 2) * lam / 2
+    p0 = np.array([1.5 * lam / 2, -2.0 * lam / 2, 30 * lam])
+    pos = np.zeros((1, 4, n_tx), np.float32)
+    pos[0, :3, :] = p0[:, None]
+    pos[0, 3, :] = 1.0
+    M = dict(M, scattererPos=pos)
+    th = np.deg2rad(np.linspace(-15, 15, n_tx))
+    for r in RX:
+        r["Apod"] = np.ones(n_ele)
+    for t, a in zip(TX, th):
+        t["Apod"] = np.ones(n_ele)
+        t["Delay"] = xe * np.sin(a) / c - (xe * np.sin(a) / c).min()
+    raw = SimUS_CUDARCA(M, TT, TR, TX, RX, device=dev)
+    raw_ref = S.simulate(M, TT, TR, TX, RX)
+    assert H.rel_l2(raw, raw_ref) <= REL
+    # time origin of the simulated RF as the beamformer sees it: the sample
+    # index of a delay tau is fs (s tau + tg) - the wrapper's frequency grid
+    # spacing fs/(2(Nf-1)) stretches time by s = Nf/(Nf-1), and the spectra
+    # add the group delays of the pulse and transmit response and subtract
+    # those of the conjugated receive response and input filter
+    P = S.prepare(M, TT, TR, TX, RX)
+    s = P["nf"] / (P["nf"] - 1)
+    tg = (np.size(TX[0]["pulseShape"]) - 1) / 2 / 250e6 + 16 / 250e6 - 16 / 250e6 - 2 / P["fs"]
+    t0 = (s - 1) * 2 * p0[2] / c - s * P["t_start"] + tg
+    gx = p0[0] + (np.arange(9) - 4) * lam / 2
+    gy = p0[1] + (np.arange(9) - 4) * lam / 2
+    gz = p0[2] + (np.arange(13) - 6) * lam / 4
+    Z, X, Y = np.meshgrid(gz, gx, gy, indexing="ij")
+    G = {"gridBfX": X, "gridBfY": Y, "gridBfZ": Z, "posEleX": np.zeros(n_ele), "posEleY": xe,
+         "posEleZ": np.zeros(n_ele)}
+    T = {"thetaX": th, "thetaY": np.zeros(n_tx), "flagReceiveX": False}
+    Par = {"samplingFrequency": P["fs"] / P["sub"], "demodulationFrequency": fc, "speedOfSound": c,
+           "startTime": t0, "fnumber": 1.0}
+    Op = {"interpolation": "lanczos4", "flagCompounding": True, "flagMaskTransmit": False}
+    vol, _ = bfVirtualSourceGeneric_RCA(raw[:, :, None, :], G, T, Par, Op, device=dev)
+    ref, _ = O.bf(raw_ref[:, :, None, :], G, T, Par, Op)
+    r = H.rel_l2(vol, ref)
+    say(f"GPU chain vs oracle chain: rel-L2 {r:.3e}")
+    assert r <= 1e-5
+    v = np.abs(vol[..., 0, 0])
+    peak = np.unravel_index(np.argmax(v), v.shape)
+    say(f"peak voxel {peak}, scatterer voxel (6, 4, 4), centre/max {v[6, 4, 4] / v.max():.3f}")
+    assert all(abs(a - b) <= 1 for a, b in zip(peak, (6, 4, 4)))
+    assert v[6, 4, 4] >= 0.9 * v.max()
