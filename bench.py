@@ -349,16 +349,34 @@ def cpu_baseline(cfg, target_s=6.0):
         return t
 
     probe = 2048
+    run(probe)                                   # warm-up (page-in, thread pool)
     tp = run(probe)
-    nvox = int(min(cfg.n_vox, max(probe, probe * target_s / max(tp, 1e-3) / 3)))
-    times = [run(nvox) for _ in range(3)]
+    nvox = int(min(cfg.n_vox, max(probe, probe * target_s / max(tp, 1e-3) / 5)))
+    times = [run(nvox) for _ in range(5)]
     t = statistics.median(times)
     upd = nvox * cfg.n_tilt * cfg.n_elem
+    # one-core figure on a smaller sample (about 1 s per run)
+    cores_all = cores
+    cores = 1
+    t1p = run(256)
+    nv1 = int(max(256, 256 * 1.0 / max(t1p, 1e-3)))
+    t1 = statistics.median([run(nv1) for _ in range(3)])
+    cores = cores_all
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": upd / t, "unit": "voxel*tilt*element updates/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "cpu_model": model,
+            "one_core_value": nv1 * cfg.n_tilt * cfg.n_elem / t1,
             "sample": f"{nvox} of {cfg.n_vox} voxels (evenly spaced), all 32 tilts x 128 elements, "
                       f"1 frame; FP32 CPU restatement of bfVirtualSourceGeneric_RCA.cu, OpenMP "
-                      f"{cores} threads, median of 3 runs ({statistics.median(times):.2f} s)"}
+                      f"{cores} threads, median of 5 runs after a warm-up ({t:.2f} s); one-core "
+                      f"figure on {nv1} voxels, median of 3"}
 
 
 def main():
