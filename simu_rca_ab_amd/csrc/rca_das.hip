@@ -1008,7 +1008,7 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
     // the bounds put every tap inside the row; the clamp only keeps LDS
     // accesses inside it
     const int off = min(max(base - cb, 0), wcap - NT);
-    float* rv = rw + off;
+    float2* rv = reinterpret_cast<float2*>(rw) + off;
     float br[FB], bi[FB];
 #pragma unroll
     for (int fi = 0; fi < FB; ++fi) {
@@ -1028,17 +1028,12 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
     if (!dup) {
 #pragma unroll
       for (int kk = 0; kk < NT; ++kk) {
-        float vr[FB], vi[FB];
+        float2 v[FB];
 #pragma unroll
-        for (int fi = 0; fi < FB; ++fi) {
-          vr[fi] = rv[fi * 2 * wcap + kk];
-          vi[fi] = rv[fi * 2 * wcap + wcap + kk];
-        }
+        for (int fi = 0; fi < FB; ++fi) v[fi] = rv[fi * wcap + kk];
 #pragma unroll
-        for (int fi = 0; fi < FB; ++fi) {
-          rv[fi * 2 * wcap + kk] = fmaf(q[kk], br[fi], vr[fi]);
-          rv[fi * 2 * wcap + wcap + kk] = fmaf(q[kk], bi[fi], vi[fi]);
-        }
+        for (int fi = 0; fi < FB; ++fi)
+          rv[fi * wcap + kk] = make_float2(fmaf(q[kk], br[fi], v[fi].x), fmaf(q[kk], bi[fi], v[fi].y));
         __asm__ volatile("" ::: "memory");
       }
     } else {
@@ -1046,8 +1041,8 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
       for (int fi = 0; fi < FB; ++fi) {
 #pragma unroll
         for (int kk = 0; kk < NT; ++kk) {
-          atomicAdd(rv + fi * 2 * wcap + kk, q[kk] * br[fi]);
-          atomicAdd(rv + fi * 2 * wcap + wcap + kk, q[kk] * bi[fi]);
+          atomicAdd(&rv[fi * wcap + kk].x, q[kk] * br[fi]);
+          atomicAdd(&rv[fi * wcap + kk].y, q[kk] * bi[fi]);
         }
       }
     }
@@ -1076,7 +1071,7 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
   const int ns = a.n_sample;
   const int wcap = a.wcap;
   const int tcn = a.tc;   // tilts per chunk (<= TC)
-  // per wave: rows [j][fi][re|im][wcap]; then per wave a claim slot per sample
+  // per wave: rows [j][fi][wcap] of complex samples; then per wave a claim slot per sample
   float* rows = reinterpret_cast<float*>(smem_raw + (size_t)tcn * kThreads * sizeof(TxTerm));
   const size_t wrows = (size_t)tcn * FB * 2 * wcap;   // floats per wave
   float* myrows = rows + wave * wrows;
@@ -1140,11 +1135,8 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
         __syncthreads();
         if (rj < nt && rfi < nf) {
           for (int w = 0; w < 4; ++w) {
-            float* r = rows + w * wrows + (size_t)row * 2 * wcap;
-            for (int i = li; i < len; i += G) {
-              r[i] = 0.0f;
-              r[wcap + i] = 0.0f;
-            }
+            float2* r = reinterpret_cast<float2*>(rows + w * wrows + (size_t)row * 2 * wcap);
+            for (int i = li; i < len; i += G) r[i] = make_float2(0.0f, 0.0f);
           }
         }
         if (ab_tc && k == 0 && tid < nt) {
@@ -1184,13 +1176,14 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
         // add the four waves' rows to the gradient traces (other tiles add to
         // the same samples)
         if (rj < nt && rfi < nf) {
-          const float* r = rows + (size_t)row * 2 * wcap;
+          const float2* r = reinterpret_cast<const float2*>(rows + (size_t)row * 2 * wcap);
+          const size_t wr2 = wrows / 2;   // complex samples per wave
           float* dst = reinterpret_cast<float*>(
               a.gin + (((long long)(f0 + rfi) * a.n_tx + t0 + rj) * a.n_ch + c) * ns + cb);
           for (int i = li; i < len; i += G) {
-            const float vr = r[i] + r[wrows + i] + r[2 * wrows + i] + r[3 * wrows + i];
-            const float vi = r[wcap + i] + r[wrows + wcap + i] + r[2 * wrows + wcap + i] +
-                             r[3 * wrows + wcap + i];
+            const float2 v0 = r[i], v1 = r[wr2 + i], v2 = r[2 * wr2 + i], v3 = r[3 * wr2 + i];
+            const float vr = v0.x + v1.x + v2.x + v3.x;
+            const float vi = v0.y + v1.y + v2.y + v3.y;
             if (vr != 0.0f) unsafeAtomicAdd(dst + 2 * i, vr);
             if (vi != 0.0f) unsafeAtomicAdd(dst + 2 * i + 1, vi);
           }
@@ -1360,7 +1353,7 @@ int pick_v(long long, int, int, int, bool) {
 }
 
 // Adjoint layout: per tilt of a chunk 256 transmit terms and, for each of the
-// 4 waves, FB rows of complex FP32 accumulators (two float planes); then a
+// 4 waves, FB rows of complex FP32 accumulators (interleaved re, im); then a
 // claim slot per row sample for each wave.  Depth tiles (64 depths x 4
 // columns) use rows of kAdjTileRow samples, other tiles whole traces; rows
 // are capped so one tilt fits 64 KB (longer windows take several passes).
