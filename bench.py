@@ -194,6 +194,74 @@ def run_svd(dev, n_frame=200, reps=3):
                         "(rocBLAS ZGEMM)"}
 
 
+def run_variants(cfg, dev, reps=3):
+    """The beamformer variants on one config-1 half (16 tilts x 128 lines,
+    FP32, one frame, Lanczos a=4): compounded output with each receive
+    aberration law (per line, per (tilt, line), per (line, voxel) with both
+    arrays), and the NoSum per-line output (the per-line output of the full
+    grid would be 69 GB: a 64-deep 128x32 slab instead).  Dense updates/s; the
+    plain kernel on the same half for comparison."""
+    import numpy as np
+    import torch
+    from simu_rca_ab_amd import configs as C, synth
+    from simu_rca_ab_amd.plan import RcaPlan
+    h = cfg.halves[0]
+    iq = synth.frame(cfg, device=dev)[0].to(dev).contiguous()
+
+    def plan_for(nz, ny=None, **kw):
+        x, y, z = C.grid_axes(cfg)
+        grid = C.meshgrid_zxy(x, y[:ny], z[:nz])
+        G = C.geometry(cfg, h, grid)
+        for k in ("gridBfX", "gridBfY", "gridBfZ"):
+            G[k] = G[k].ravel(order="F")
+        P = dict(C.param(cfg), **kw)
+        return RcaPlan.from_reference(G, C.transmit(h), P, C.options(cfg), n_sample=cfg.n_sample,
+                                      device=dev, depth_run=nz)
+
+    def timeit(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    rng = np.random.default_rng(77)
+    n_ch, n_tx, nz = cfg.n_elem, len(h.theta), cfg.nz
+    res = {}
+    base = plan_for(nz)
+    upd = base.n_pix * n_tx * n_ch
+    out = torch.empty((1, 1, base.n_pix), dtype=torch.complex64, device=dev)
+    t = timeit(lambda: base.forward(iq, out=out, want_mask=False))
+    res["plain"] = {"ms": t * 1e3, "updates_per_s": upd / t}
+    laws = {"law_per_line": dict(AbLawRx=(rng.standard_normal(n_ch) * 20e-9).astype(np.float32),
+                                 AbLawMode="channel"),
+            "law_per_tilt_line": dict(AbLawRx=(rng.standard_normal(n_tx * n_ch) * 20e-9).astype(np.float32),
+                                      AbLawMode="tx_channel"),
+            # [nChannel][nPix] device tensors (2.1 GB each)
+            "law_per_line_voxel": dict(
+                AbLawRx=torch.randn((n_ch, base.n_pix), device=dev) * 20e-9,
+                AbLawTx=torch.randn((n_ch, base.n_pix), device=dev) * 20e-9,
+                AbLawMode="pixel_channel")}
+    for name, kw in laws.items():
+        pl = plan_for(nz, **kw)
+        t = timeit(lambda: pl.forward(iq, out=out, want_mask=False))
+        res[name] = {"ms": t * 1e3, "updates_per_s": upd / t, "vs_plain": (upd / t) / res["plain"]["updates_per_s"]}
+        del pl
+        torch.cuda.empty_cache()
+    del laws
+    slab = plan_for(64, ny=32)
+    n_out = torch.empty((n_ch, 1, n_tx, slab.n_pix), dtype=torch.complex64, device=dev)
+    t = timeit(lambda: slab.forward(iq, compound=False, nosum=True, out=n_out, want_mask=False))
+    u = slab.n_pix * n_tx * n_ch
+    res["nosum_slab"] = {"ms": t * 1e3, "updates_per_s": u / t, "voxels": slab.n_pix,
+                         "output_bytes": int(n_out.numel() * 8)}
+    res["workload"] = ("cfg1 half 0 (128 lines, 16 tilts, 2048 samples, Lanczos a=4, FP32, 1 frame); "
+                       "laws on 128x128x256 voxels, NoSum on a 64-deep 128x32 slab (262144 voxels)")
+    return res
+
+
 def sim_case(n_elem=128, n_sub=128, n_tilt=21, n_scat=500, n_sample=1024, fc=7.5e6, seed=4242):
     """SIMUS-RCA workload of one half of the dataset generator
     (save_simus_sh.m:44-110, 193-225): 128 row elements transmit and 128
@@ -390,6 +458,8 @@ def main():
     ap.add_argument("--no-svd", action="store_true", help="skip the SVD clutter-filter leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (cfg4) leg")
     ap.add_argument("--no-sim", action="store_true", help="skip the SIMUS-RCA simulator leg")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the NoSum / aberration-law variants leg")
     ap.add_argument("--dbf", action="store_true",
                     help="add the differentiable-beamforming leg (its first run compiles MIOpen "
                          "Conv3d kernels, which can take minutes)")
@@ -440,6 +510,10 @@ def main():
     if not args.no_svd:
         log("[bench] SVD clutter-filter leg (128x128x256 x 200 frames)")
         extra["svd_filter"] = run_svd(dev)
+        torch.cuda.empty_cache()
+    if not args.no_variants:
+        log("[bench] NoSum / aberration-law variants leg")
+        extra["variants"] = run_variants(cfg, dev)
         torch.cuda.empty_cache()
     if not args.no_sim:
         log("[bench] SIMUS-RCA simulator leg (one half-frame)")
