@@ -614,8 +614,10 @@ __device__ __forceinline__ bool line_window(const KArgs& a, const TileBox& b, do
 // CMP: compound over tilts (acc per voxel and frame).  Otherwise TC = 1 and
 // accT holds the current tilt's sums over the lines, or one line's terms with
 // nosum.  PX: the PIXEL_CHANNEL law (a separate instantiation, so the other
-// kernels keep their register budget).
-template <int KIND, int V, int FB, int TC, bool CMP, typename S, bool PX = false>
+// kernels keep their register budget).  NS: NoSum with tilt chunks - every
+// update is stored as it is computed (the output was cleared before the
+// launch), so a chunk of TC tilts shares each line's receive terms.
+template <int KIND, int V, int FB, int TC, bool CMP, typename S, bool PX = false, bool NS = false>
 __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned by,
                                           unsigned char* smem_raw) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
@@ -623,7 +625,8 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   constexpr int R = TC * FB;          // LDS rows at most
   constexpr int G = kThreads / R;     // threads per row when staging
   static_assert(R <= 64 && kThreads % R == 0, "rows must divide the workgroup");
-  static_assert(CMP || TC == 1, "per-tilt outputs take one tilt per chunk");
+  static_assert(CMP || NS || TC == 1, "per-tilt outputs take one tilt per chunk");
+  static_assert(!(CMP && NS), "NoSum outputs are not compounded");
   TxTerm* txc = reinterpret_cast<TxTerm*>(smem_raw);
   const S* iq = reinterpret_cast<const S*>(a.iq);
   __shared__ float s_box[4][6];
@@ -639,7 +642,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   const int nf = min(FB, a.n_frame - f0);
   const int ns = a.n_sample;
   const int wcap = a.wcap;
-  const int tcn = CMP ? a.tc : 1;   // tilts per chunk (<= TC)
+  const int tcn = (CMP || NS) ? a.tc : 1;   // tilts per chunk (<= TC)
   S* tr = reinterpret_cast<S*>(smem_raw + (size_t)tcn * V * kThreads * sizeof(TxTerm));
   // pass advance; at least one 16-byte chunk so the pass loop always ends
   // (rows shorter than NT + E only occur with traces too short for any tap)
@@ -655,6 +658,9 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   bool valid[V];
   const TileBox box = tile_voxels<V, kTileCols, 4>(a, bx, tid, rx, pl, pt, pz, apF, iapF, valid, s_box);
   int line_no = 0;   // lines visited, alternates the s_pxr slot
+  long long pv[V];   // NS: voxel indices for the stores
+#pragma unroll
+  for (int v = 0; v < V; ++v) pv[v] = NS ? voxel_of<kTileCols, 4>(a, bx, V, v, tid, ok) : 0;
 
   float2 acc[V][FB];
   float macc[V];
@@ -690,7 +696,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
         uabc = a.fs * ab;
         pabc = a.fd * ab;
       }
-      if (!CMP && a.nosum) {   // per-line outputs: the sums restart at every line
+      if (!CMP && !NS && a.nosum) {   // per-line outputs: the sums restart at every line
 #pragma unroll
         for (int v = 0; v < V; ++v)
 #pragma unroll
@@ -818,16 +824,25 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
                     r2 *= sc;
                     i2 *= sc;
                   }
-                  float2& dst = CMP ? acc[v][fi] : accT[v][fi];
-                  dst.x = fmaf(re, r2, fmaf(-im, i2, dst.x));
-                  dst.y = fmaf(re, i2, fmaf(im, r2, dst.y));
+                  if constexpr (NS) {   // NoSum.cu:376-380: out[c][f][t][p], one update each
+                    float2* o = a.out +
+                                (((long long)c * a.n_frame + f0 + fi) * a.n_tx + t0 + j) * a.n_pix +
+                                pv[v];
+                    float2 r = make_float2(fmaf(re, r2, -im * i2), fmaf(re, i2, im * r2));
+                    if (a.accumulate) { const float2 old = *o; r.x += old.x; r.y += old.y; }
+                    *o = r;
+                  } else {
+                    float2& dst = CMP ? acc[v][fi] : accT[v][fi];
+                    dst.x = fmaf(re, r2, fmaf(-im, i2, dst.x));
+                    dst.y = fmaf(re, i2, fmaf(im, r2, dst.y));
+                  }
                 }
               }
             }
           }
         }
       }
-      if (!CMP && a.nosum) {   // NoSum.cu:228-229,275-278,376-380: out[c][f][t][p]
+      if (!CMP && !NS && a.nosum) {   // NoSum.cu:228-229,275-278,376-380: out[c][f][t][p]
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           if (!valid[v]) continue;
@@ -845,7 +860,17 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
       }
     }
 
-    if (!CMP) {   // per-tilt output and mask of tilt t0
+    if (NS) {   // masks of the chunk's tilts: the line sum for insonified tilts
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        if (!valid[v] || !a.mask || by != 0) continue;
+        for (int j = 0; j < nt; ++j) {
+          float* mk = a.mask + (long long)(t0 + j) * a.n_pix + pv[v];
+          const float mv = ((act[v] >> j) & 1u) ? mT[v] : 0.0f;
+          *mk = a.accumulate ? *mk + mv : mv;
+        }
+      }
+    } else if (!CMP) {   // per-tilt output and mask of tilt t0
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         if (!valid[v]) continue;
@@ -901,11 +926,11 @@ __host__ __device__ constexpr int tc_max(int fb, int sbytes) {
   return r > RCA_TC_MAX ? RCA_TC_MAX : (r < 1 ? 1 : r);
 }
 
-template <int KIND, int V, int FB, bool CMP, typename S, bool PX>
+template <int KIND, int V, int FB, bool CMP, typename S, bool PX, bool NS>
 __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  fwd_block<KIND, V, FB, CMP ? tc_max(FB, sizeof(S)) : 1, CMP, S, PX>(a, blockIdx.x, blockIdx.y,
-                                                                     smem_raw);
+  fwd_block<KIND, V, FB, (CMP || NS) ? tc_max(FB, sizeof(S)) : 1, CMP, S, PX, NS>(
+      a, blockIdx.x, blockIdx.y, smem_raw);
 }
 
 // ----------------------------------------------------------------------------
@@ -1284,9 +1309,9 @@ constexpr size_t kFwdLds = 48 * 1024, kFwdLds1 = 32 * 1024;
 
 template <typename S> struct SName;
 
-template <int KIND, int V, int FB, bool CMP, typename S, bool PX = false>
+template <int KIND, int V, int FB, bool CMP, typename S, bool PX = false, bool NS = false>
 int launch_fwd_t(KArgs a, hipStream_t st) {
-  const FwdLayout L = fwd_layout(a, V, FB, sizeof(S), CMP ? tc_max(FB, sizeof(S)) : 1,
+  const FwdLayout L = fwd_layout(a, V, FB, sizeof(S), (CMP || NS) ? tc_max(FB, sizeof(S)) : 1,
                                  g_tune_lds ? (size_t)g_tune_lds * 1024 : FB == 1 ? kFwdLds1 : kFwdLds);
   if (L.lds > (size_t)kMaxLds)
     return fail(RCA_ERR_UNSUPPORTED, "FB=%d traces of %d samples do not fit in LDS", FB, a.n_sample);
@@ -1295,7 +1320,8 @@ int launch_fwd_t(KArgs a, hipStream_t st) {
   const unsigned gx = (unsigned)fwd_blocks(a.n_pix, a.depth_run, V);
   const unsigned gy = (unsigned)((a.n_frame + FB - 1) / FB);
   if (gx == 0 || gy == 0) return RCA_OK;
-  hipLaunchKernelGGL((das_fwd<KIND, V, FB, CMP, S, PX>), dim3(gx, gy, 1), dim3(kThreads), L.lds, st, a);
+  hipLaunchKernelGGL((das_fwd<KIND, V, FB, CMP, S, PX, NS>), dim3(gx, gy, 1), dim3(kThreads), L.lds, st,
+                     a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "das_fwd launch");
 }
@@ -1318,6 +1344,9 @@ int launch_fwd_k(const KArgs& a, int V, int FB, hipStream_t st) {
     if (V == VV && FB == FF) return launch_fwd_t<KIND, VV, FF, true, S>(a, st);
     RCA_FWD_CASE(1, 1) RCA_FWD_CASE(2, 1) RCA_FWD_CASE(1, 2) RCA_FWD_CASE(1, 4) RCA_FWD_CASE(1, 8)
 #undef RCA_FWD_CASE
+  } else if (a.nosum) {   // NoSum with tilt chunks (output cleared by the caller)
+    if (V == 1 && FB == 1) return launch_fwd_t<KIND, 1, 1, false, S, false, true>(a, st);
+    if (V == 1 && FB == 4) return launch_fwd_t<KIND, 1, 4, false, S, false, true>(a, st);
   } else {
 #define RCA_FWD_CASE(FF) \
     if (V == 1 && FB == FF) return launch_fwd_t<KIND, 1, FF, false, S>(a, st);
@@ -1469,6 +1498,13 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
   if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) {   // the instantiated pixel-law shapes
     v = 1;
     fb = fb >= 4 ? 4 : 1;
+  }
+  if (a.nosum && !a.accumulate && d->ab_law_mode != RCA_ABLAW_PIXEL_CHANNEL) {
+    // the tilt-chunked NoSum kernel stores only the updates it computes
+    const long long n = 2LL * d->n_channel * n_frame * d->n_tx * d->n_pix;
+    hipLaunchKernelGGL(fill_zero, dim3(4096), dim3(256), 0, stream, reinterpret_cast<float*>(out), n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "fill_zero launch");
   }
   if (iq_dtype == RCA_IQ_C64) return launch_fwd_s<float2>(a, d->interp, v, fb, stream);
   return launch_fwd_s<__half2>(a, d->interp, v, fb, stream);
