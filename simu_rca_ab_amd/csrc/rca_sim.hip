@@ -38,6 +38,7 @@
 #include <tuple>
 
 #include "../../include/rca_das.h"
+#include "rca_math.h"
 #include "../../include/rca_sim.h"
 
 namespace rca_internal {
@@ -46,6 +47,9 @@ void clear_error();
 }  // namespace rca_internal
 
 namespace {
+
+using rca_math::sincos2pi;
+using rca_math::sqrt_nr;
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -63,12 +67,9 @@ int hip_fail(hipError_t e, const char* what) {
 
 constexpr int kThreads = 256;
 #ifndef RCA_SIM_F
-#define RCA_SIM_F 4
+#define RCA_SIM_F 4   // frequencies per lane: 4 measured best (2: -3 %, 8: -21 %)
 #endif
-constexpr int kF = RCA_SIM_F;                    // frequencies per lane
-#ifndef RCA_SIM_WAVES
-#define RCA_SIM_WAVES 0
-#endif
+constexpr int kF = RCA_SIM_F;
 constexpr float kPiRef = 3.14159265359f;         // the kernels' float PI (SimUS_CUDARCA.cu:8)
 constexpr double kInv2Pi = 0.15915494309189533577;
 
@@ -81,42 +82,6 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
-
-// sqrt in FP64 from the hardware reciprocal square root and a Newton pair
-// (Goldschmidt form); x > 0.
-__device__ __forceinline__ double sqrt_nr(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
-  double g = x * y;
-  double h = 0.5 * y;
-  const double r = fma(-g, h, 0.5);
-  g = fma(g, r, g);
-  h = fma(h, r, h);
-  const double d = fma(-g, g, x);
-  return fma(d, h, g);
-}
-
-// sin(2 pi p), cos(2 pi p) for |p| <= 1/2: Taylor polynomials of the half
-// angle (|pi p| <= pi/2) and the double-angle identities.
-__device__ __forceinline__ void sincos2pi(float p, float& s, float& c) {
-  const float t = p * 3.14159265358979f;
-  const float t2 = t * t;
-  float sh = fmaf(t2, -1.60590438e-10f, 2.50521084e-8f);
-  sh = fmaf(t2, sh, -2.75573192e-6f);
-  sh = fmaf(t2, sh, 1.98412698e-4f);
-  sh = fmaf(t2, sh, -8.33333333e-3f);
-  sh = fmaf(t2, sh, 1.66666667e-1f);
-  sh = t * fmaf(-t2, sh, 1.0f);
-  float ch = fmaf(t2, 4.77947733e-14f, -1.14707456e-11f);
-  ch = fmaf(t2, ch, 2.08767570e-9f);
-  ch = fmaf(t2, ch, -2.75573192e-7f);
-  ch = fmaf(t2, ch, 2.48015873e-5f);
-  ch = fmaf(t2, ch, -1.38888889e-3f);
-  ch = fmaf(t2, ch, 4.16666667e-2f);
-  ch = fmaf(t2, ch, -0.5f);
-  ch = fmaf(t2, ch, 1.0f);
-  s = 2.0f * sh * ch;
-  c = fmaf(-2.0f * sh, sh, 1.0f);
-}
 
 // exp(2 pi j x) for a phase given in cycles (FP64), reduced first.
 __device__ __forceinline__ float2 cis_cycles(double x) {
@@ -341,9 +306,6 @@ struct TxArgs {
   int n_scat, n_f, n_tx;
 };
 
-#if RCA_SIM_WAVES
-__attribute__((amdgpu_waves_per_eu(RCA_SIM_WAVES)))
-#endif
 __global__ void __launch_bounds__(kThreads) sim_tx(const TxArgs a) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -397,9 +359,6 @@ struct RxfArgs {
   int n_scat, n_f;
 };
 
-#if RCA_SIM_WAVES
-__attribute__((amdgpu_waves_per_eu(RCA_SIM_WAVES)))
-#endif
 __global__ void __launch_bounds__(kThreads) sim_rxfield(const RxfArgs a) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -437,9 +396,6 @@ struct RxArgs {
   int n_scat, n_f, n_tx, nfft;
 };
 
-#if RCA_SIM_WAVES
-__attribute__((amdgpu_waves_per_eu(RCA_SIM_WAVES)))
-#endif
 __global__ void __launch_bounds__(kThreads) sim_rx(const RxArgs a) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
