@@ -185,3 +185,42 @@ def test_tiled_adjoint_per_tilt_and_accumulate(dev):
     say(f"per-tilt tiled vs untiled: rel-L2 {r:.3e}")
     assert r <= TOL
     assert H.rel_l2(c.cpu().numpy(), 2 * a_t.cpu().numpy()) <= 1e-6
+
+
+def test_nosum_tilt_chunks_frames_mask_fp16_accumulate(dev):
+    """NoSum takes tilt chunks and stores each update into a cleared output:
+    5 frames (frame blocks of 4 + 1), the transmit mask on (zeros and mask 0
+    for tilts that do not insonify a voxel), FP16-I/Q, accumulation."""
+    say("case NoSum tiled: 5 frames, transmit mask, FP16, accumulate vs FP64 oracle")
+    G0, T, P, Op = H.case(TILE, 1)
+    # a 4 mm square aperture: its tilted cone leaves out the deep voxels of the
+    # steep tilts (with the receive-axis span alone the cone is empty)
+    G0 = dict(G0, apertureX=0.004, apertureY=0.004, apertureZ=0.0)
+    Op = dict(Op, interpolation="lanczos4", flagMaskTransmit=True)
+    kw = dict(n_sample=TILE.n_sample, device=dev)
+    plan = RcaPlan.from_reference(H.flat_geometry(G0), T, P, Op, depth_run=TILE.nz, **kw)
+    assert plan.depth_run == TILE.nz
+    iq = H.iq_frame(TILE, 1, n_frame=5)
+    x = iq.to(dev)
+    out, mask = plan.forward(x, compound=False, nosum=True)
+    torch.cuda.synchronize()
+    ref, rmask = O.kernel_forward(iq.numpy().view(np.float32), H.grid_f32(G0), O.prepare(G0, T),
+                                  H.oracle_params(TILE, TILE.halves[1]), O.INTERP_LANCZOS, 4, True,
+                                  nosum=True)
+    # the oracle leaves outputs of uninsonified (voxel, tilt) untouched (zeros here)
+    r = H.rel_l2(out.cpu().numpy(), ref)
+    say(f"NoSum 5 frames + mask: rel-L2 {r:.3e}")
+    assert r <= 1e-5
+    assert H.rel_l2(mask.cpu().numpy(), rmask) <= 1e-5
+    m = mask.cpu().numpy()
+    assert (m == 0).any() and (m > 0).any()          # the cone excludes some (voxel, tilt)
+    acc = out.clone()
+    plan.forward(x, compound=False, nosum=True, out=acc, want_mask=False, accumulate=True)
+    torch.cuda.synchronize()
+    assert H.rel_l2(acc.cpu().numpy(), 2 * out.cpu().numpy()) <= 1e-6
+    h, s = quantize_fp16(x)
+    o16, _ = plan.forward(h, s, compound=False, nosum=True, want_mask=False)
+    torch.cuda.synchronize()
+    r16 = H.rel_l2(o16.cpu().numpy(), ref)
+    say(f"NoSum FP16: rel-L2 {r16:.3e}")
+    assert r16 <= 3e-3
