@@ -195,6 +195,41 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_bl
  */
 int rca_set_tuning(int v, int fb, int split, int lds_kib);
 
+/*
+ * Resident plans (the boundary's rca_plan_create / rca_plan_destroy): the
+ * host prep of bfVirtualSourceGeneric_RCA.m:491-558 in FP64 and one upload
+ * of the geometry, so a C or MATLAB caller needs nothing but host arrays.
+ * Arrays are host memory in MATLAB's column-major flatten; values are
+ * rounded to FP32 as the wrapper's single() does (RCA.m:598-636).
+ */
+typedef struct {
+  const double *grid_x, *grid_y, *grid_z;   /* [nPix] gridBfX/Y/Z (m) */
+  const double *ele_x, *ele_y, *ele_z;      /* [nChannel] posEleX/Y/Z, as given (centred here) */
+  const double *theta_x, *theta_y;          /* [nTx] tilt angles (rad), or both NULL ... */
+  const double *vs_x, *vs_y, *vs_z;         /* ... and the virtual sources given instead */
+  const double *ab_law_rx, *ab_law_tx;      /* aberration laws (s), layouts of rca_ablaw, or NULL */
+  int n_pix, n_channel, n_tx, n_sample;
+  int receive_x, mask_transmit, interp, ab_law_mode;
+  int depth_run;                            /* as in rca_das_desc (0 = none) */
+  int has_aperture;                         /* 0: apertures = span of the centred elements */
+  double aperture_x, aperture_y, aperture_z;
+  double sampling_frequency, demodulation_frequency, speed_of_sound, start_time, fnumber;
+} rca_plan_params;
+
+typedef struct rca_plan rca_plan;
+
+/* Host-only prep (no GPU): centred elements [3][nChannel], virtual sources
+ * [3][nTx] (tilts -> r = 10 m, RCA.m:527-541), minDist [nTx] with the
+ * reference's X/Y swap (RCA.m:551-558) and apertures [3], all FP32. */
+int rca_prepare_geometry(const rca_plan_params* p, float* ele, float* vs, float* min_dist,
+                         float* aperture);
+
+/* Allocate and upload a plan on the current device (the only allocation;
+ * rca_forward / rca_adjoint on its descriptor allocate nothing). */
+int rca_plan_create(const rca_plan_params* p, rca_plan** plan);
+const rca_das_desc* rca_plan_desc(const rca_plan* plan);
+int rca_plan_destroy(rca_plan* plan);
+
 /* Last error message of the calling thread ("" when none). */
 const char* rca_last_error(void);
 

@@ -22,7 +22,8 @@ ABLAW = {None: 0, "channel": 1, "tx_channel": 2, "pixel_channel": 3}
 EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last_error",
            "rca_version", "rca_set_tuning", "rca_batch_workspace_bytes", "rca_batch_encode",
            "rca_batch_launch", "rca_svd_workspace_bytes", "rca_svd_filter", "rca_sim_txfield",
-           "rca_sim_rxfield", "rca_sim_receive", "rca_sim_spectra_to_raw")
+           "rca_sim_rxfield", "rca_sim_receive", "rca_sim_spectra_to_raw", "rca_prepare_geometry",
+           "rca_plan_create", "rca_plan_desc", "rca_plan_destroy")
 
 
 class RcaDasDesc(ctypes.Structure):
@@ -35,6 +36,17 @@ class RcaDasDesc(ctypes.Structure):
         "aperture_x", "aperture_y", "aperture_z", "sampling_frequency",
         "demodulation_frequency", "speed_of_sound", "start_time", "fnumber")] + [
         ("depth_run", ctypes.c_int), ("ab_law_tx", ctypes.c_void_p)]
+
+
+class RcaPlanParams(ctypes.Structure):
+    """Mirror of ``rca_plan_params``."""
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "grid_x", "grid_y", "grid_z", "ele_x", "ele_y", "ele_z", "theta_x", "theta_y", "vs_x",
+        "vs_y", "vs_z", "ab_law_rx", "ab_law_tx")] + [(n, ctypes.c_int) for n in (
+            "n_pix", "n_channel", "n_tx", "n_sample", "receive_x", "mask_transmit", "interp",
+            "ab_law_mode", "depth_run", "has_aperture")] + [(n, ctypes.c_double) for n in (
+                "aperture_x", "aperture_y", "aperture_z", "sampling_frequency",
+                "demodulation_frequency", "speed_of_sound", "start_time", "fnumber")]
 
 
 class RcaSimReceiveArgs(ctypes.Structure):
@@ -77,10 +89,17 @@ def load():
     lib.rca_sim_txfield.argtypes = [vp] * 8 + [f] * 6 + [i] * 5 + [vp, vp]
     lib.rca_sim_rxfield.argtypes = [vp] * 5 + [f] * 6 + [i] * 4 + [vp]
     lib.rca_sim_receive.argtypes = [ctypes.POINTER(RcaSimReceiveArgs), vp]
+    pp = ctypes.POINTER(RcaPlanParams)
+    lib.rca_prepare_geometry.argtypes = [pp, vp, vp, vp, vp]
+    lib.rca_plan_create.argtypes = [pp, ctypes.POINTER(vp)]
+    lib.rca_plan_desc.argtypes = [vp]
+    lib.rca_plan_desc.restype = dp
+    lib.rca_plan_destroy.argtypes = [vp]
     lib.rca_sim_spectra_to_raw.argtypes = [vp, i, i, i, i, vp, vp]
     for name in ("rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA",
                  "rca_svd_filter", "rca_sim_txfield", "rca_sim_rxfield", "rca_sim_receive",
-                 "rca_sim_spectra_to_raw"):
+                 "rca_sim_spectra_to_raw", "rca_prepare_geometry", "rca_plan_create",
+                 "rca_plan_destroy"):
         getattr(lib, name).restype = ctypes.c_int
     lib.rca_last_error.restype = ctypes.c_char_p
     lib.rca_version.restype = ctypes.c_char_p

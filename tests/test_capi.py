@@ -62,7 +62,8 @@ def test_svd_validation_without_gpu():
 
 def header_struct_fields(name, header="rca_das.h"):
     text = re.sub(r"/\*.*?\*/", "", (INCLUDE / header).read_text(), flags=re.S)
-    body = re.search(r"typedef struct \{(.*?)\}\s*" + name + ";", text, flags=re.S).group(1)
+    body = re.search(r"typedef struct \{((?:(?!typedef struct).)*?)\}\s*" + name + ";", text,
+                     flags=re.S).group(1)
     fields = []
     for decl in body.split(";"):
         decl = decl.strip()
@@ -76,10 +77,12 @@ def header_struct_fields(name, header="rca_das.h"):
 
 
 @pytest.mark.parametrize("cls,name,header", [(_lib.RcaDasDesc, "rca_das_desc", "rca_das.h"),
+                                             (_lib.RcaPlanParams, "rca_plan_params", "rca_das.h"),
                                              (_lib.RcaSimReceiveArgs, "rca_sim_receive_args",
                                               "rca_sim.h")])
 def test_struct_mirrors_match_headers(cls, name, header):
-    kinds = {ctypes.c_void_p: "ptr", ctypes.c_int: "int", ctypes.c_float: "float"}
+    kinds = {ctypes.c_void_p: "ptr", ctypes.c_int: "int", ctypes.c_float: "float",
+             ctypes.c_double: "double"}
     mirror = [(n, kinds[t]) for n, t in cls._fields_]
     assert mirror == header_struct_fields(name, header)
 
@@ -117,3 +120,89 @@ def test_ablaw_validation_without_gpu(mode, msg):
     rc = lib.rca_adjoint(ctypes.byref(d), None, 1, 1, None, 0, None)
     assert rc == _lib.RCA_ERR_INVALID
     assert msg in lib.rca_last_error()
+
+
+def _plan_params(G, T, cfg, keep):
+    """rca_plan_params of a Geometry/Transmit pair (host FP64 arrays)."""
+    import numpy as np
+    p = _lib.RcaPlanParams()
+
+    def arr(x):
+        a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel(order="F"))
+        keep.append(a)
+        return a.ctypes.data
+
+    p.grid_x, p.grid_y, p.grid_z = (arr(G[k]) for k in ("gridBfX", "gridBfY", "gridBfZ"))
+    p.ele_x, p.ele_y, p.ele_z = (arr(G[k]) for k in ("posEleX", "posEleY", "posEleZ"))
+    if "thetaX" in T:
+        p.theta_x, p.theta_y = arr(T["thetaX"]), arr(T["thetaY"])
+        p.n_tx = np.size(T["thetaX"])
+    else:
+        p.vs_x, p.vs_y, p.vs_z = (arr(T[k]) for k in ("virtualSourceX", "virtualSourceY", "virtualSourceZ"))
+        p.n_tx = np.size(T["virtualSourceX"])
+    p.n_pix = np.size(G["gridBfX"])
+    p.n_channel = np.size(G["posEleX"])
+    p.n_sample = cfg.n_sample
+    p.receive_x = int(bool(T.get("flagReceiveX", True)))
+    if "apertureX" in G:
+        p.has_aperture = 1
+        p.aperture_x, p.aperture_y, p.aperture_z = G["apertureX"], G["apertureY"], G["apertureZ"]
+    p.sampling_frequency, p.demodulation_frequency = cfg.fs, cfg.fd
+    p.speed_of_sound, p.start_time, p.fnumber = cfg.c, cfg.t0, cfg.fnumber
+    return p
+
+
+@pytest.mark.parametrize("half_index,sources,apertures", [(0, False, True), (1, False, False),
+                                                          (0, True, False)])
+def test_prepare_geometry_matches_host_prep(half_index, sources, apertures):
+    """The C host prep equals the Python drop-in's prep (RCA.m:491-558)."""
+    import numpy as np
+    from simu_rca_ab_amd import configs as C
+    from simu_rca_ab_amd.plan import prepare_geometry
+    from tests import helpers as H
+    cfg = C.scaled(C.get("cfg1"), nx=4, ny=3, nz=8)
+    G, T, P, Op = H.case(cfg, half_index)
+    G = dict(G, posEleX=np.asarray(G["posEleX"]) + 3e-4, posEleZ=np.asarray(G["posEleZ"]) + 1e-5)
+    if not apertures:
+        G = {k: v for k, v in G.items() if not k.startswith("aperture")}
+    if sources:
+        pr = prepare_geometry(G, T)
+        T = {"virtualSourceX": pr["vs"][0] * 1.0, "virtualSourceY": pr["vs"][1] * 1.0,
+             "virtualSourceZ": pr["vs"][2] * 1.0, "flagReceiveX": T["flagReceiveX"]}
+    want = prepare_geometry(G, T)
+    keep = []
+    p = _plan_params(G, T, cfg, keep)
+    nc, nt = p.n_channel, p.n_tx
+    ele = np.zeros(3 * nc, np.float32)
+    vs = np.zeros(3 * nt, np.float32)
+    d0 = np.zeros(nt, np.float32)
+    ap = np.zeros(3, np.float32)
+    lib = _lib.load()
+    rc = lib.rca_prepare_geometry(ctypes.byref(p), ele.ctypes.data, vs.ctypes.data, d0.ctypes.data,
+                                  ap.ctypes.data)
+    assert rc == 0, lib.rca_last_error()
+    # equal to the last bit, or one FP32 ulp where the FP64 sums round differently
+    np.testing.assert_allclose(ele.reshape(3, nc), np.stack(want["ele"]), rtol=2e-7, atol=1e-12)
+    np.testing.assert_allclose(vs.reshape(3, nt), np.stack(want["vs"]), rtol=2e-7, atol=1e-12)
+    np.testing.assert_allclose(d0, want["d0"], rtol=2e-7)
+    np.testing.assert_allclose(ap, want["aperture"], rtol=2e-7)
+
+
+def test_plan_validation_without_gpu():
+    lib = _lib.load()
+    plan = ctypes.c_void_p()
+    assert lib.rca_plan_create(None, ctypes.byref(plan)) == _lib.RCA_ERR_INVALID
+    p = _lib.RcaPlanParams()
+    p.n_pix, p.n_channel, p.n_tx, p.n_sample = 10, 4, 2, 64
+    assert lib.rca_plan_create(ctypes.byref(p), ctypes.byref(plan)) == _lib.RCA_ERR_INVALID
+    assert b"element" in lib.rca_last_error()
+    import numpy as np
+    e = np.zeros(4)
+    th = np.zeros(2)
+    p.ele_x = p.ele_y = p.ele_z = e.ctypes.data
+    assert lib.rca_plan_create(ctypes.byref(p), ctypes.byref(plan)) == _lib.RCA_ERR_INVALID
+    assert b"either" in lib.rca_last_error()     # neither tilts nor sources
+    p.theta_x = p.theta_y = p.vs_x = p.vs_y = p.vs_z = th.ctypes.data
+    assert lib.rca_plan_create(ctypes.byref(p), ctypes.byref(plan)) == _lib.RCA_ERR_INVALID
+    assert b"either" in lib.rca_last_error()     # both
+    assert lib.rca_plan_destroy(None) == 0
