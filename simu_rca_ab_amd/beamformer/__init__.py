@@ -19,7 +19,7 @@ import warnings
 import numpy as np
 import torch
 
-from ..plan import RcaPlan, interpolation_name, prepare_geometry
+from ..plan import RcaPlan, interpolation_name
 
 __all__ = ["bfVirtualSourceGeneric_RCA", "bfVirtualSourceGeneric_RCA_Nosum",
            "bfVirtualSourceGeneric_RCA_ab_1", "bfVirtualSourceGeneric_RCA_ab_2",

@@ -14,7 +14,6 @@ one call per frame).
 from __future__ import annotations
 
 import numpy as np
-import torch
 
 from .beamformer import bfVirtualSourceGeneric_RCA_Nosum, bfVirtualSourceGeneric_RCA_Nosum_ab
 

@@ -16,7 +16,6 @@ Tensor layouts (C-order):
 from __future__ import annotations
 
 import ctypes
-import math
 
 import numpy as np
 import torch

@@ -17,7 +17,7 @@ import torch
 
 from oracle import rca_oracle as O
 from simu_rca_ab_amd import _lib, configs as C
-from simu_rca_ab_amd.plan import RcaPlan, dequantize_fp16, quantize_fp16
+from simu_rca_ab_amd.plan import dequantize_fp16, quantize_fp16
 from tests import helpers as H
 from tests.test_gpu_forward import check, oracle_fwd, plan_for, say
 

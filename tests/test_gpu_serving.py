@@ -5,7 +5,6 @@ import time
 
 import numpy as np
 import pytest
-import torch
 
 from oracle import rca_oracle as O
 from simu_rca_ab_amd import configs as C

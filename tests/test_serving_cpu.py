@@ -1,6 +1,5 @@
 """Scheduler / admission logic of the serving loop on the CPU, through a fake
 backend with the GPU backend's interface (no GPU work)."""
-import threading
 import time
 
 import numpy as np
