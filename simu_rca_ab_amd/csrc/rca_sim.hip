@@ -18,10 +18,12 @@
 //   sim_tx       TX per (tilt, scatterer, frequency); lanes = scatterers, each
 //                wave a block of F frequencies, so element positions,
 //                wavenumbers and laws are wave-uniform (scalar registers)
-//   sim_rx       the fused receive: lanes = receive elements, each wave a
-//                frequency block; per scatterer (wave-uniform position and TX
-//                value) the sub-element sum, accumulated against TX; writes
-//                the conjugated, zero-padded spectra (and the aberrated ones)
+//   sim_rx       the fused receive: lanes = receive elements, a workgroup a
+//                frequency block whose four waves take every fourth
+//                scatterer (wave-uniform position and TX value): the
+//                sub-element sum accumulated against TX, the four partial
+//                sums added in LDS; writes the conjugated, zero-padded
+//                spectra (and the aberrated ones)
 //   sim_rxfield  the reference RXfield output, materialised (reference ABI)
 //   sim_raw      real part / N, trim or pad, sample-mode decimation
 // The inverse FFT is hipFFT.  Distances and every phase (k r, w Delay,
@@ -397,10 +399,12 @@ struct RxArgs {
 };
 
 __global__ void __launch_bounds__(kThreads) sim_rx(const RxArgs a) {
+  // the four waves of a workgroup share one frequency block and take every
+  // fourth scatterer; their sums meet in LDS (fixed order)
+  __shared__ float2 s_part[3][kF][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int f0 = (blockIdx.y * 4 + wave) * kF;
-  if (f0 >= a.n_f) return;
+  const int f0 = blockIdx.y * kF;
   const int t = blockIdx.z;
   const int ne = a.T.n_ele;
   const int e = blockIdx.x * 64 + lane;
@@ -411,7 +415,7 @@ __global__ void __launch_bounds__(kThreads) sim_rx(const RxArgs a) {
   float2 acc[kF];
 #pragma unroll
   for (int i = 0; i < kF; ++i) acc[i] = make_float2(0.0f, 0.0f);
-  for (int s = 0; s < a.n_scat; ++s) {
+  for (int s = wave; s < a.n_scat; s += 4) {
     float2 tv[kF];
     bool any = false;
 #pragma unroll
@@ -426,7 +430,15 @@ __global__ void __launch_bounds__(kThreads) sim_rx(const RxArgs a) {
 #pragma unroll
     for (int i = 0; i < kF; ++i) acc[i] = cadd(acc[i], cmul(tv[i], h[i]));
   }
-  if (e >= ne) return;
+  if (wave > 0) {
+#pragma unroll
+    for (int i = 0; i < kF; ++i) s_part[wave - 1][i][lane] = acc[i];
+  }
+  __syncthreads();
+  if (wave > 0 || e >= ne) return;
+#pragma unroll
+  for (int i = 0; i < kF; ++i)
+    acc[i] = cadd(cadd(acc[i], s_part[0][i][lane]), cadd(s_part[1][i][lane], s_part[2][i][lane]));
   const double lc = a.T.lens * a.T.c;
   const float ap = a.apod[(long long)t * ne + e];
   const long long row = ((long long)t * ne + e) * a.nfft;
@@ -591,7 +603,10 @@ int rca_sim_receive(const rca_sim_receive_args* p, hipStream_t stream) {
                          p->nfft);
   }
   if (p->n_f > 0)
-    hipLaunchKernelGGL(sim_rx, field_grid(p->n_ele, p->n_f, p->n_tx), dim3(kThreads), 0, stream, a);
+    hipLaunchKernelGGL(sim_rx,
+                       dim3((unsigned)((p->n_ele + 63) / 64), (unsigned)((p->n_f + kF - 1) / kF),
+                            (unsigned)p->n_tx),
+                       dim3(kThreads), 0, stream, a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "sim_rx launch");
 }
