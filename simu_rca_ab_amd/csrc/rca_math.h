@@ -45,6 +45,28 @@ __device__ __forceinline__ void sincos2pi(float p, float& s, float& c) {
   c = fmaf(-2.0f * sh, sh, 1.0f);
 }
 
+// The same with the half-angle series one term shorter (sine to t^11, cosine
+// to t^12): truncation below 6e-8 (sine) and 7e-9 (cosine) at |t| = pi/2,
+// about one FP32 ulp after the double angle.  Three FMAs fewer, for kernels
+// that evaluate it per term (the simulator).
+__device__ __forceinline__ void sincos2pi_lean(float p, float& s, float& c) {
+  const float t = p * 3.14159265358979f;
+  const float t2 = t * t;
+  float sh = fmaf(t2, 2.50521084e-8f, -2.75573192e-6f);
+  sh = fmaf(t2, sh, 1.98412698e-4f);
+  sh = fmaf(t2, sh, -8.33333333e-3f);
+  sh = fmaf(t2, sh, 1.66666667e-1f);
+  sh = t * fmaf(-t2, sh, 1.0f);
+  float ch = fmaf(t2, 2.08767570e-9f, -2.75573192e-7f);
+  ch = fmaf(t2, ch, 2.48015873e-5f);
+  ch = fmaf(t2, ch, -1.38888889e-3f);
+  ch = fmaf(t2, ch, 4.16666667e-2f);
+  ch = fmaf(t2, ch, -0.5f);
+  ch = fmaf(t2, ch, 1.0f);
+  s = 2.0f * sh * ch;
+  c = fmaf(-2.0f * sh, sh, 1.0f);
+}
+
 }  // namespace rca_math
 
 #endif  // RCA_MATH_H

@@ -50,7 +50,7 @@ void clear_error();
 
 namespace {
 
-using rca_math::sincos2pi;
+using rca_math::sincos2pi_lean;
 using rca_math::sqrt_nr;
 
 int fail(int code, const char* fmt, ...) {
@@ -88,7 +88,7 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 // exp(2 pi j x) for a phase given in cycles (FP64), reduced first.
 __device__ __forceinline__ float2 cis_cycles(double x) {
   float s, c;
-  sincos2pi((float)(x - rint(x)), s, c);
+  sincos2pi_lean((float)(x - rint(x)), s, c);
   return make_float2(c, s);
 }
 
@@ -96,13 +96,15 @@ __device__ __forceinline__ float2 cis_cycles(double x) {
 // sin(a + jb) = sin a cosh b + j cos a sinh b.  |a| < pi here, so sin a and
 // cos a come from the half-cycle polynomial; cosh and sinh of small b from
 // their series (no cancellation: two terms below 1e-2, where the next is
-// under 1e-9 relative; five below 0.3), of larger b from exp.
+// under 1e-9 relative; five below 0.3), of larger b from exp.  The division
+// by x uses the hardware reciprocal (1 ulp) of |x|^2; below |x|^2 = 1e-12
+// the value is 1 (the exact one differs by |x|^2/6 < 2e-13).
 __device__ __forceinline__ float2 csinc(float a, float b) {
   const float m2 = fmaf(a, a, b * b);
-  if (!(m2 > 0.0f)) return make_float2(1.0f, 0.0f);
+  if (!(m2 > 1e-12f)) return make_float2(1.0f, 0.0f);
   if (!(m2 < kPiRef * kPiRef)) return make_float2(0.0f, 0.0f);
   float sa, ca;
-  sincos2pi(a * 0.159154943091895f, sa, ca);
+  sincos2pi_lean(a * 0.159154943091895f, sa, ca);
   float ch, sh;
   const float ab = fabsf(b);
   if (ab < 1e-2f) {
@@ -120,7 +122,7 @@ __device__ __forceinline__ float2 csinc(float a, float b) {
     sh = 0.5f * (e - ie);
   }
   const float nr = sa * ch, ni = ca * sh;
-  const float inv = 1.0f / m2;
+  const float inv = __builtin_amdgcn_rcpf(m2);
   return make_float2((nr * a + ni * b) * inv, (ni * a - nr * b) * inv);
 }
 
@@ -187,9 +189,11 @@ __device__ __forceinline__ Geo geometry(double sx, double sy, double sz, const f
   const double dx = sx - (double)p[0], dy = sy - (double)p[n_ele], dz = sz - (double)p[2 * n_ele];
   g.r = sqrt_nr(fma(dx, dx, fma(dy, dy, dz * dz)));
   const float rf = (float)g.r;
-  g.ir = 1.0f / rf;
+  g.ir = __builtin_amdgcn_rcpf(rf);   // 1 ulp
   const float ux = (float)dx * g.ir, uy = (float)dy * g.ir;
-  const float cx = sqrtf(fmaxf(1.0f - ux * ux, 0.0f)), cy = sqrtf(fmaxf(1.0f - uy * uy, 0.0f));
+  // hardware square root (1 ulp): the correctly rounded one costs a sequence
+  const float cx = __builtin_amdgcn_sqrtf(fmaxf(1.0f - ux * ux, 0.0f));
+  const float cy = __builtin_amdgcn_sqrtf(fmaxf(1.0f - uy * uy, 0.0f));
   const float az0 = p[3 * n_ele], el0 = p[4 * n_ele];
   if (az0 == 0.0f) {
     g.saz = ux;
