@@ -171,7 +171,16 @@ template <> struct Interp<RCA_INTERP_LANCZOS4> { static constexpr int LO = 3, HI
 // one small-angle sincos serves all taps, and the tap nearest to u (j = 0)
 // is evaluated from e directly (series near 0), which keeps every weight
 // accurate to a few ulp even when d rounds to 0 or 1.
-template <int KIND>
+//
+// The 1/x^2 factors cost a hardware reciprocal each, a quarter-rate
+// instruction.  With BATCH, one reciprocal of their product serves all
+// (prefix products forward, then two multiplies per factor backward: 3(n-1)
+// full-rate multiplies for n-1 reciprocals saved; the factors lie in
+// [1e-12, 20.25], the product well inside FP32 range, relative error a few
+// ulp).  It holds up to eight more VGPRs, so kernels near the 128-VGPR
+// occupancy step (frame blocks, pixel laws, NoSum) keep the per-tap
+// reciprocals.
+template <int KIND, bool BATCH = false>
 __device__ __forceinline__ void interp_weights(float d, float* q) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI;
   if constexpr (KIND == RCA_INTERP_QUADRATIC) {
@@ -196,25 +205,45 @@ __device__ __forceinline__ void interp_weights(float d, float* q) {
     const float Ks = K * se, Kc = K * ce;
     constexpr int J0 = -LO - 1;
     constexpr int NJ = LO + HI + 2;
-    float wj[NJ];
-#pragma unroll
-    for (int i = 0; i < NJ; ++i) {
+    const float e2 = e * e;
+    // weight of tap index i (j = J0 + i) given 1/x^2
+    auto weight = [&](int i, float inv) {
       const int j = J0 + i;
       if (j == 0) {
-        const float e2 = e * e;
         constexpr float a2 = kPi2 / 6.0f * (1.0f + 1.0f / (W * W));
         constexpr float a4 = kPi2 * kPi2 * (1.0f / 120.0f + 1.0f / (120.0f * W * W * W * W) +
                                             1.0f / (36.0f * W * W));
         const float ser = fmaf(e2, fmaf(e2, a4, -a2), 1.0f);
-        const float rat = K * se * __builtin_amdgcn_rcpf(e2);
-        wj[i] = fabsf(e) < 1e-3f ? ser : rat;
-      } else {
-        const float x = e - (float)j;
-        const float sg = (j & 1) ? -1.0f : 1.0f;
-        const float cj = sg * cs_cos(W, j), sj = -sg * cs_sin(W, j);
-        const float num = sj == 0.0f ? Ks * cj : cj == 0.0f ? Kc * sj : fmaf(Ks, cj, Kc * sj);
-        wj[i] = num * __builtin_amdgcn_rcpf(x * x);
+        return fabsf(e) < 1e-3f ? ser : K * se * inv;
       }
+      const float sg = (j & 1) ? -1.0f : 1.0f;
+      const float cj = sg * cs_cos(W, j), sj = -sg * cs_sin(W, j);
+      const float num = sj == 0.0f ? Ks * cj : cj == 0.0f ? Kc * sj : fmaf(Ks, cj, Kc * sj);
+      return num * inv;
+    };
+    // x^2 of every tap, the centre one clamped away from 0 (below |e| = 1e-3
+    // its weight comes from the series)
+    float x2[NJ], wj[NJ];
+#pragma unroll
+    for (int i = 0; i < NJ; ++i) {
+      const float x = e - (float)(J0 + i);
+      x2[i] = J0 + i == 0 ? fmaxf(e2, 1e-12f) : x * x;
+    }
+    if constexpr (BATCH) {
+      float pre[NJ];
+      pre[0] = x2[0];
+#pragma unroll
+      for (int i = 1; i < NJ; ++i) pre[i] = pre[i - 1] * x2[i];
+      float r = __builtin_amdgcn_rcpf(pre[NJ - 1]);   // 1 / (x2[0] ... x2[i]) below
+#pragma unroll
+      for (int i = NJ - 1; i > 0; --i) {
+        wj[i] = weight(i, r * pre[i - 1]);
+        r *= x2[i];
+      }
+      wj[0] = weight(0, r);
+    } else {
+#pragma unroll
+      for (int i = 0; i < NJ; ++i) wj[i] = weight(i, __builtin_amdgcn_rcpf(x2[i]));
     }
 #pragma unroll
     for (int k = -LO; k <= HI; ++k) q[k + LO] = up ? wj[k - 1 - J0] : wj[k - J0];
@@ -762,7 +791,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
               const int base = m - LO;
               if ((k > 0 && base < cb) || (!last && base >= cb + step)) continue;
               float q[NT];
-              interp_weights<KIND>(d, q);
+              interp_weights<KIND, FB == 1 && !PX && !NS>(d, q);
               float pr = er * x.et.x - ei * x.et.y;
               float pi = er * x.et.y + ei * x.et.x;
               if (ab_tc) {
