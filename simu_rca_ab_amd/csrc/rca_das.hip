@@ -102,13 +102,12 @@ struct KArgs {
 // Device math.
 // ----------------------------------------------------------------------------
 
-// sin(pi x), cos(pi x) for |x| <= 1/(2W) (the Lanczos tap identities take
-// x = e/W with |e| <= 1/2).  Taylor series truncated where the next term is
-// below 2e-8 at the interval's end: W = 2 (|pi x| <= pi/4) to x^9 / x^10,
-// W = 3 (<= pi/6) to x^7 / x^8, W = 4 (<= pi/8) to x^7 / x^6.
+// sin t, cos t for |t| <= pi/(2W) (the Lanczos tap identities take
+// t = pi e/W with |e| <= 1/2).  Taylor series truncated where the next term is
+// below 2e-8 at the interval's end: W = 2 (|t| <= pi/4) to t^9 / t^10,
+// W = 3 (<= pi/6) to t^7 / t^8, W = 4 (<= pi/8) to t^7 / t^6.
 template <int W>
-__device__ __forceinline__ void sincospi_small(float x, float& s, float& c) {
-  const float t = x * 3.14159265358979f;
+__device__ __forceinline__ void sincos_small(float t, float& s, float& c) {
   const float t2 = t * t;
   if constexpr (W <= 2) {
     s = t * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.75573192e-6f, -1.98412698e-4f), 8.33333333e-3f),
@@ -194,7 +193,7 @@ __device__ __forceinline__ void interp_weights(float d, float* q) {
     const bool up = d >= 0.5f;
     const float e = up ? d - 1.0f : d;
     float se, ce;
-    sincospi_small<W>(e * (1.0f / (float)W), se, ce);
+    sincos_small<W>(e * (3.14159265358979f / (float)W), se, ce);
     float S;  // sin(pi e) from the multiple-angle identity
     if constexpr (W == 2) S = 2.0f * se * ce;
     else if constexpr (W == 3) S = se * fmaf(-4.0f * se, se, 3.0f);
