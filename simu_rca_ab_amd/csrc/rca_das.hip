@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -1228,7 +1229,15 @@ __global__ void fill_zero(float* __restrict__ p, long long n) {
 // ----------------------------------------------------------------------------
 // Host side.
 // ----------------------------------------------------------------------------
-int g_tune_v = 0, g_tune_fb = 0, g_tune_lds = 0;
+// process-wide launch-shape overrides (rca_set_tuning); atomics, so a sweep
+// thread may change them while other threads launch
+std::atomic<int> g_tune_v{0}, g_tune_fb{0}, g_tune_lds{0};
+
+// LDS budget of a workgroup: the override when set, else the default
+size_t lds_budget(size_t dflt) {
+  const int kib = g_tune_lds.load();
+  return kib ? (size_t)kib * 1024 : dflt;
+}
 
 int check_desc(const rca_das_desc* d) {
   if (!d) return fail(RCA_ERR_INVALID, "desc is NULL");
@@ -1307,7 +1316,7 @@ template <typename S> struct SName;
 template <int KIND, int V, int FB, bool CMP, typename S, bool PX = false, bool NS = false>
 int launch_fwd_t(KArgs a, hipStream_t st) {
   const FwdLayout L = fwd_layout(a, V, FB, sizeof(S), (CMP || NS) ? tc_max(FB, sizeof(S)) : 1,
-                                 g_tune_lds ? (size_t)g_tune_lds * 1024 : FB == 1 ? kFwdLds1 : kFwdLds);
+                                 lds_budget(FB == 1 ? kFwdLds1 : kFwdLds));
   if (L.lds > (size_t)kMaxLds)
     return fail(RCA_ERR_UNSUPPORTED, "FB=%d traces of %d samples do not fit in LDS", FB, a.n_sample);
   a.tc = L.tc;
@@ -1400,7 +1409,7 @@ AdjLayout adj_layout(const KArgs& a, int fb, int tc_kernel, size_t budget) {
 
 template <int KIND, int FB, bool PX = false>
 int launch_adj_t(KArgs a, hipStream_t st) {
-  const AdjLayout L = adj_layout(a, FB, tc_max(FB, 8), g_tune_lds ? (size_t)g_tune_lds * 1024 : kAdjLds);
+  const AdjLayout L = adj_layout(a, FB, tc_max(FB, 8), lds_budget(kAdjLds));
   if (L.lds > (size_t)kMaxLds || L.wcap < Interp<KIND>::LO + Interp<KIND>::HI + 1)
     return fail(RCA_ERR_UNSUPPORTED, "FB=%d adjoint rows do not fit in LDS", FB);
   a.tc = L.tc;
@@ -1482,9 +1491,10 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
   a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
   a.nosum = (flags & RCA_FLAG_NOSUM) != 0;
   a.vec_ok = (((uintptr_t)iq & 15) == 0) && (((size_t)d->n_sample * sbytes) % 16 == 0);
-  int fb = g_tune_fb ? g_tune_fb : pick_fb(n_frame, d->n_sample, sbytes, a.depth_run != 0);
+  const int tfb = g_tune_fb.load(), tv = g_tune_v.load();
+  int fb = tfb ? tfb : pick_fb(n_frame, d->n_sample, sbytes, a.depth_run != 0);
   const int nfb = (n_frame + fb - 1) / fb;
-  int v = g_tune_v ? g_tune_v : pick_v(d->n_pix, a.depth_run, nfb, fb, a.compound != 0);
+  int v = tv ? tv : pick_v(d->n_pix, a.depth_run, nfb, fb, a.compound != 0);
   if (v > 2) v = 2;                                  // instantiated shapes
   if (!a.compound || fb > 1) v = 1;
   if (!a.compound && fb > 1) fb = fb >= 4 ? 4 : 1;
@@ -1522,7 +1532,8 @@ int rca_adjoint(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_
   a.gout_tx = grad_tx;
   a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
   // one frame per workgroup measured fastest (more tilts per chunk fit LDS)
-  int fb = g_tune_fb ? g_tune_fb : 1;
+  const int tfb = g_tune_fb.load();
+  int fb = tfb ? tfb : 1;
   while (fb > 1 && fb > n_frame) fb >>= 1;
   if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) fb = 1;   // the instantiated pixel-law shape
   // every workgroup adds its windows to the traces: start from zero

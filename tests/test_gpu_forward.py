@@ -169,6 +169,30 @@ def test_cfg1_full_size_sampled_voxels(dev):
     check(got_all[sel], ref, "cfg1 full (3000 sampled voxels)")
 
 
+def test_cfg1_full_size_linearity_and_tilt_sum(dev):
+    """Size-independent properties on the full config-1 half (8.6e9 updates
+    per launch): the output is linear in the I/Q, and the compounded volume
+    and mask equal the sums of the per-tilt ones."""
+    cfg = C.get("cfg1")
+    say("case cfg1 full half 0: linearity A(2x - 0.5j y) and compound == sum of per-tilt outputs")
+    x = synth.frame(cfg)[0].to(dev)
+    g = torch.Generator(device="cpu").manual_seed(7)
+    y = torch.complex(torch.randn(x.shape, generator=g), torch.randn(x.shape, generator=g)).to(dev)
+    plan, _, _ = plan_for(cfg, 0, dev)
+    ax, mask = plan.forward(x)
+    ay, _ = plan.forward(y)
+    az, _ = plan.forward((2.0 * x - 0.5j * y).contiguous())
+    r = H.rel_l2(az.cpu().numpy(), (2.0 * ax - 0.5j * ay).cpu().numpy())
+    say(f"linearity: rel-L2 {r:.3e}")
+    assert r <= 1e-5
+    per, pmask = plan.forward(x, compound=False)
+    assert per.shape[1] == len(cfg.halves[0].theta)
+    r = H.rel_l2(per.sum(dim=1).cpu().numpy(), ax[:, 0].cpu().numpy())
+    rm = H.rel_l2(pmask.sum(dim=0).cpu().numpy(), mask[0].cpu().numpy())
+    say(f"tilt sum: rel-L2 {r:.3e}, mask {rm:.3e}")
+    assert r <= REL and rm <= REL
+
+
 def test_golden_fixture(dev):
     from tests.golden import make_golden as MG
     say("case golden fixture rca_cfg0_tiny")

@@ -194,3 +194,28 @@ def test_simulated_point_focuses_through_beamformer(dev):
     say(f"peak voxel {peak}, scatterer voxel (6, 4, 4), centre/max {v[6, 4, 4] / v.max():.3f}")
     assert all(abs(a - b) <= 1 for a, b in zip(peak, (6, 4, 4)))
     assert v[6, 4, 4] >= 0.9 * v.max()
+
+
+def test_superposition_and_scaling_full_probe(dev):
+    """Size-independent properties on the full probe of the dataset generator
+    (128 + 128 elements x 128 sub-elements, 21 tilts, 640 frequencies): the
+    output is additive over scatterer sets, and doubling every amplitude
+    doubles it (a power-of-two scale commutes with FP32 rounding, so the
+    difference is expected to be exactly zero)."""
+    import bench
+    say("case full probe 128+128 x 128 sub-elements, 21 tilts: Raw(A u B) = Raw(A) + Raw(B); Raw(2a) = 2 Raw(a)")
+    M, TT, TR, TX, RX = bench.sim_case(n_scat=48)
+    pos = M["scattererPos"]
+    raw = SimUS_CUDARCA(M, TT, TR, TX, RX, device=dev)
+    ra = SimUS_CUDARCA(dict(M, scattererPos=np.ascontiguousarray(pos[:20])), TT, TR, TX, RX, device=dev)
+    rb = SimUS_CUDARCA(dict(M, scattererPos=np.ascontiguousarray(pos[20:])), TT, TR, TX, RX, device=dev)
+    r = H.rel_l2(ra + rb, raw)
+    say(f"superposition: rel-L2 {r:.3e}  shape {raw.shape}")
+    assert np.isfinite(raw).all() and np.abs(raw).max() > 0
+    assert r <= 1e-5
+    p2 = pos.copy()
+    p2[:, 3, :] *= 2
+    raw2 = SimUS_CUDARCA(dict(M, scattererPos=p2), TT, TR, TX, RX, device=dev)
+    d = np.abs(raw2 - 2 * raw).max()
+    say(f"scaling: max |Raw(2a) - 2 Raw(a)| = {d:.3e} (bitwise equal: {d == 0})")
+    assert d <= 1e-7 * np.abs(raw).max()
