@@ -160,12 +160,15 @@ int rca_adjoint(const rca_das_desc* desc, const void* grad_out, int grad_tx, int
  * (complex64 [n_frame][1][nPix]) and `mask` ([1][nPix], written by the batch
  * that contains frame 0).  rca_batch_encode fills a host workspace (header,
  * per-half arguments, work items); the caller copies it to a device buffer of
- * the same size; rca_batch_launch runs a persistent grid of grid_blocks
- * workgroups over the device work list.  The launch arguments never change
+ * the same size; rca_batch_launch runs the batch as four launches (FP32
+ * items' first halves, their second halves, then the same for FP16) of
+ * max_items workgroups, one work item per workgroup (max_items as given to
+ * rca_batch_encode).  The launch arguments never change
  * between batches, so one captured HIP graph replays every batch.  Per-voxel
  * arithmetic is the same as rca_forward's: results are bitwise identical
- * whatever else shares the batch.  FP16 entries need frame_begin even and
- * frame_end even or equal to n_frame.
+ * whatever else shares the batch.  Frames are processed in blocks of
+ * rca_batch_frame_block(iq_dtype): an entry's frame_begin must be a multiple
+ * of it, and frame_end a multiple of it or n_frame.
  */
 typedef struct {
   const rca_das_desc* half[2];
@@ -183,8 +186,11 @@ typedef struct {
 size_t rca_batch_workspace_bytes(int max_halves, int max_items);
 int rca_batch_encode(const rca_batch_entry* entries, int n_entries, int max_halves, int max_items,
                      int n_sample_max, void* host_ws, size_t ws_bytes, int* n_items);
-int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_blocks,
+int rca_batch_launch(const void* dev_ws, int max_halves, int max_items, int interp,
                      int n_sample_max, hipStream_t stream);
+/* Frames per block of a batch entry of this I/Q dtype (1 for complex64); 0
+ * for an unknown dtype. */
+int rca_batch_frame_block(int iq_dtype);
 
 /*
  * Launch-shape override for tuning sweeps (no reference counterpart):

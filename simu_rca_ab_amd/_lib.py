@@ -21,7 +21,7 @@ ABLAW = {None: 0, "channel": 1, "tx_channel": 2, "pixel_channel": 3}
 # Every symbol include/rca_das.h, include/rca_svd.h and include/rca_sim.h declare.
 EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last_error",
            "rca_version", "rca_set_tuning", "rca_batch_workspace_bytes", "rca_batch_encode",
-           "rca_batch_launch", "rca_svd_workspace_bytes", "rca_svd_filter", "rca_sim_txfield",
+           "rca_batch_launch", "rca_batch_frame_block", "rca_svd_workspace_bytes", "rca_svd_filter", "rca_sim_txfield",
            "rca_sim_rxfield", "rca_sim_receive", "rca_sim_spectra_to_raw", "rca_prepare_geometry",
            "rca_plan_create", "rca_plan_desc", "rca_plan_destroy")
 

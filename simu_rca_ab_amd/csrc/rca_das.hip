@@ -930,15 +930,21 @@ __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
 }
 
 // ----------------------------------------------------------------------------
-// Batched forward for the serving loop.  A persistent grid walks a work list
-// kept in device memory (so one captured launch serves batches of any shape):
-//   header  { n_items }
+// Batched forward for the serving loop.  The work list is kept in device
+// memory, so one captured sequence of launches serves batches of any shape:
+//   header  { n FP32 items, n FP16 items }
 //   args    KArgs per request half (pointers, sizes, flags; iq_dtype below)
-//   items   { first arg, number of halves (1|2), voxel block, frame block }
-// Workgroup g takes items g, g + gridDim.x, ...  Both halves of an item run in
-// the same workgroup, the second accumulating onto the first, so items never
-// race on an output.  FP32 items use V = kBatchV32, FB = 1; FP16 items
-// V = 1, FB = kBatchFB16.
+//   items   { first arg, number of halves (1|2), voxel block, frame block },
+//           the FP32 items first, then the FP16 items
+// Four launches per batch: FP32 half 0, FP32 half 1, FP16 half 0, FP16 half 1.
+// Workgroup g of a launch takes item g of its type (a grid of max_items
+// workgroups; those past the item count return at once), so the second half
+// of an item accumulates onto the first in the next launch and no two
+// workgroups of a launch share an output.  One kernel per sample type and no
+// loop around the block keep each at the register count of the plain kernels
+// (a persistent kernel over both types and halves took 160 VGPRs, 3 waves per
+// SIMD, and ran FP32 items at two thirds of the plain kernel's rate).  FP32
+// items use V = kBatchV32, FB = 1; FP16 items V = 1, FB = kBatchFB16.
 // ----------------------------------------------------------------------------
 struct BatchArgs {
   KArgs k;
@@ -946,26 +952,42 @@ struct BatchArgs {
   int pad_[3];
 };
 
+// FP16 frames per workgroup in batches (rca_batch_frame_block).
+#ifndef RCA_BATCH_FB16
+#define RCA_BATCH_FB16 4
+#endif
 constexpr int kBatchV32 = 1;
-constexpr int kBatchFB16 = 2;
+constexpr int kBatchFB16 = RCA_BATCH_FB16;
 
-template <int KIND>
+// A wave-uniform struct read from memory, word by word through
+// readfirstlane: the values land in scalar registers, as kernel arguments do
+// (loaded as vector values they took about 30 VGPRs more).
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  T v;
+  const int* src = reinterpret_cast<const int*>(p);
+  int* dst = reinterpret_cast<int*>(&v);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return v;
+}
+
+template <int KIND, typename S>
 __global__ void __launch_bounds__(kThreads) das_fwd_batch(const int* __restrict__ header,
                                                           const BatchArgs* __restrict__ args,
-                                                          const int4* __restrict__ items) {
+                                                          const int4* __restrict__ items, int h) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int n = header[0];
-  for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    const int4 it = items[i];
-    for (int h = 0; h < it.y; ++h) {
-      const KArgs k = args[it.x + h].k;
-      if (args[it.x + h].iq_dtype == RCA_IQ_C64)
-        fwd_block<KIND, kBatchV32, 1, tc_max(1, 8), true, float2>(k, it.z, it.w, smem_raw);
-      else
-        fwd_block<KIND, 1, kBatchFB16, tc_max(kBatchFB16, 4), true, __half2>(k, it.z, it.w, smem_raw);
-      __syncthreads();
-    }
-  }
+  constexpr bool kF32 = sizeof(S) == 8;
+  const int n = kF32 ? header[0] : header[1];
+  if ((int)blockIdx.x >= n) return;
+  const int4 it = uniform_load(items + (kF32 ? 0 : header[0]) + blockIdx.x);
+  if (h >= it.y) return;
+  const KArgs k = uniform_load(&args[it.x + h].k);
+  if constexpr (kF32)
+    fwd_block<KIND, kBatchV32, 1, tc_max(1, 8), true, float2>(k, it.z, it.w, smem_raw);
+  else
+    fwd_block<KIND, 1, kBatchFB16, tc_max(kBatchFB16, 4), true, __half2>(k, it.z, it.w, smem_raw);
 }
 
 // ----------------------------------------------------------------------------
@@ -1615,12 +1637,16 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
   unsigned char* ws = reinterpret_cast<unsigned char*>(host_ws);
   BatchArgs* args = reinterpret_cast<BatchArgs*>(ws + batch_args_off());
   int4* items = reinterpret_cast<int4*>(ws + batch_items_off(max_halves));
-  int na = 0, ni = 0;
+  int na = 0, ni = 0, n32 = 0;
+  // the FP32 entries' items first (pass 0), then the FP16 entries' (pass 1)
+  for (int q = 0; q < n_entries; ++q)
+    if (e[q].iq_dtype != RCA_IQ_C64 && e[q].iq_dtype != RCA_IQ_F16X2)
+      return fail(RCA_ERR_INVALID, "entry %d: unknown iq dtype", q);
+  for (int pass = 0; pass < 2; ++pass)
   for (int q = 0; q < n_entries; ++q) {
     const rca_batch_entry& E = e[q];
+    if ((E.iq_dtype == RCA_IQ_C64) != (pass == 0)) continue;
     if (E.n_half < 1 || E.n_half > 2) return fail(RCA_ERR_INVALID, "entry %d: n_half must be 1 or 2", q);
-    if (E.iq_dtype != RCA_IQ_C64 && E.iq_dtype != RCA_IQ_F16X2)
-      return fail(RCA_ERR_INVALID, "entry %d: unknown iq dtype", q);
     const int fb = E.iq_dtype == RCA_IQ_C64 ? 1 : kBatchFB16;
     if (!E.out || E.n_frame < 1 || E.frame_begin < 0 || E.frame_end > E.n_frame ||
         E.frame_begin >= E.frame_end || E.frame_begin % fb ||
@@ -1668,18 +1694,20 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
       for (long long vb = 0; vb < nvb; ++vb) {
         if (ni >= max_items) return fail(RCA_ERR_UNSUPPORTED, "batch exceeds %d items", max_items);
         items[ni++] = make_int4(first, E.n_half, (int)vb, fbk);
+        if (pass == 0) ++n32;
       }
     }
   }
-  reinterpret_cast<int*>(ws)[0] = ni;
+  reinterpret_cast<int*>(ws)[0] = n32;
+  reinterpret_cast<int*>(ws)[1] = ni - n32;
   *n_items_out = ni;
   return RCA_OK;
 }
 
-int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_blocks,
+int rca_batch_launch(const void* dev_ws, int max_halves, int max_items, int interp,
                      int n_sample_max, hipStream_t stream) {
   g_err.clear();
-  if (!dev_ws || max_halves < 1 || grid_blocks < 1 || n_sample_max < 1)
+  if (!dev_ws || max_halves < 1 || max_items < 1 || n_sample_max < 1)
     return fail(RCA_ERR_INVALID, "bad batch launch arguments");
   const size_t lds = batch_lds(n_sample_max);
   if (lds > (size_t)kMaxLds) return fail(RCA_ERR_UNSUPPORTED, "nSample %d exceeds LDS", n_sample_max);
@@ -1687,9 +1715,15 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_bl
   const int* hdr = reinterpret_cast<const int*>(ws);
   const BatchArgs* args = reinterpret_cast<const BatchArgs*>(ws + batch_args_off());
   const int4* items = reinterpret_cast<const int4*>(ws + batch_items_off(max_halves));
+  const dim3 grid(max_items), block(kThreads);
   switch (interp) {
-#define RCA_BATCH_CASE(K) \
-    case K: hipLaunchKernelGGL(das_fwd_batch<K>, dim3(grid_blocks), dim3(kThreads), lds, stream, hdr, args, items); break;
+#define RCA_BATCH_CASE(K)                                                                       \
+    case K:                                                                                     \
+      for (int h = 0; h < 2; ++h)                                                               \
+        hipLaunchKernelGGL((das_fwd_batch<K, float2>), grid, block, lds, stream, hdr, args, items, h); \
+      for (int h = 0; h < 2; ++h)                                                               \
+        hipLaunchKernelGGL((das_fwd_batch<K, __half2>), grid, block, lds, stream, hdr, args, items, h); \
+      break;
     RCA_BATCH_CASE(RCA_INTERP_QUADRATIC)
     RCA_BATCH_CASE(RCA_INTERP_LANCZOS_REF)
     RCA_BATCH_CASE(RCA_INTERP_LANCZOS3)
@@ -1699,6 +1733,10 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int interp, int grid_bl
   }
   const hipError_t err = hipGetLastError();
   return err == hipSuccess ? RCA_OK : hip_fail(err, "das_fwd_batch launch");
+}
+
+int rca_batch_frame_block(int iq_dtype) {
+  return iq_dtype == RCA_IQ_C64 ? 1 : iq_dtype == RCA_IQ_F16X2 ? kBatchFB16 : 0;
 }
 
 }  // extern "C"

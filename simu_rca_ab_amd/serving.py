@@ -15,7 +15,9 @@ FP32 scale).  The server
   attained service first, so requests that arrive while others run join the
   next batch ahead of long-running ones;
 * launches every batch as a replay of one of two captured HIP graphs of the
-  persistent batched kernel (``rca_batch_launch``), whose work list is copied
+  batched kernels (``rca_batch_launch``: one workgroup per work item, FP32
+  and FP16 items in kernels of their own, an item's second RCA half in a
+  launch after its first), whose work list is copied
   into a fixed device buffer right before the replay — two work-list slots
   keep one batch queued while the previous one runs;
 * downloads each finished request's volume and records its latency
@@ -124,7 +126,7 @@ class _RcaBatchEntry(ctypes.Structure):
 class GpuBackend:
     """Resident geometry, HBM allocations, streams and the captured batch graphs."""
 
-    def __init__(self, device, n_elem=128, n_sample=2048, interp="lanczos4", grid_blocks=1024,
+    def __init__(self, device, n_elem=128, n_sample=2048, interp="lanczos4",
                  max_halves=256, max_items=1 << 17, fs=C.FS, fd=C.F0, c=C.C_SOUND, t0=0.0,
                  fnumber=1.0, dxyz=C.LAMBDA / 2, z0=5e-3, max_tilt_deg=12.0):
         self.lib = _lib.load()
@@ -138,9 +140,12 @@ class GpuBackend:
         lib.rca_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_void_p]
         lib.rca_batch_launch.restype = ctypes.c_int
+        lib.rca_batch_frame_block.argtypes = [ctypes.c_int]
+        # FP16 frames run in blocks of this many per workgroup: chunks start on a block
+        self.fb16 = int(lib.rca_batch_frame_block(_lib.IQ_F16X2))
         self.device = torch.device(device)
         self.n_elem, self.n_sample, self.interp = n_elem, n_sample, interp
-        self.grid_blocks, self.max_halves, self.max_items = grid_blocks, max_halves, max_items
+        self.max_halves, self.max_items = max_halves, max_items
         self.fs, self.fd, self.c, self.t0, self.fnumber = fs, fd, c, t0, fnumber
         self.dxyz, self.z0, self.max_tilt_deg = dxyz, z0, max_tilt_deg
         self.compute = torch.cuda.Stream(self.device)
@@ -162,7 +167,7 @@ class GpuBackend:
     # -- graphs ---------------------------------------------------------
     def _launch(self, k):
         rc = self.lib.rca_batch_launch(ctypes.c_void_p(self.dev_ws[k].data_ptr()), self.max_halves,
-                                       _lib.INTERP[self.interp], self.grid_blocks, self.n_sample,
+                                       self.max_items, _lib.INTERP[self.interp], self.n_sample,
                                        ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
         _lib.check(rc)
 
@@ -296,13 +301,15 @@ class GpuBackend:
 
     def max_frames_per_chunk(self, spec: RequestSpec, quantum: int) -> int:
         """Frames per work chunk: about ``quantum`` updates, within the work-list
-        capacity; even for FP16 requests (the batched kernel runs FP16 frames
-        in pairs, so chunks must start on an even frame)."""
+        capacity; a multiple of the FP16 frame block for FP16 requests (the
+        batched kernel runs FP16 frames in blocks, so chunks must start on a
+        block)."""
         per_frame = spec.updates(self.n_elem) // spec.n_frame
-        items_per_frame = math.ceil(spec.n_vox / 512)
+        items_per_frame = math.ceil(spec.n_vox / 256)   # a work item tiles 16 depths x 16 columns
         f = max(1, min(quantum // max(per_frame, 1), self.max_items // max(items_per_frame, 1)))
         if spec.dtype == "fp16":
-            f = max(2, f - f % 2)
+            b = self.fb16
+            f = max(b, f - f % b)
         return f
 
 

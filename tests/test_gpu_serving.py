@@ -22,8 +22,8 @@ SPECS = [RequestSpec(16, 16, 32, 8, 3, "fp32"), RequestSpec(24, 20, 40, 16, 2, "
 
 @pytest.fixture(scope="module")
 def backend(dev):
-    return GpuBackend(dev, n_elem=64, n_sample=1024, interp="lanczos4", grid_blocks=256,
-                      max_halves=32, max_items=1 << 14)
+    return GpuBackend(dev, n_elem=64, n_sample=1024, interp="lanczos4", max_halves=32,
+                      max_items=1 << 14)
 
 
 @pytest.fixture(scope="module")
