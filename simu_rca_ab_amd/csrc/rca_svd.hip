@@ -151,21 +151,36 @@ __global__ void __launch_bounds__(256) gram_f64(const float2* __restrict__ iq,
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wi = w & 1, wj = w >> 1;   // the wave's 16 x 16 sub-block
   d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-  for (long long k0 = k_begin; k0 < k_end; k0 += kGK) {
-    __syncthreads();
-    for (int e = tid; e < 2 * kGT * kGK; e += 256) {
+  // software pipeline: the next slice's samples (kSlice per thread) and mask
+  // weight are loaded into registers while the MFMAs consume the current one
+  constexpr int kSlice = 2 * kGT * kGK / 256;
+  float2 pre[kSlice];
+  double wpre = 0.0;
+  auto load_slice = [&](long long k0) {
+#pragma unroll
+    for (int u = 0; u < kSlice; ++u) {
+      const int e = tid + 256 * u;
       const int f = e / kGK, q = e % kGK;   // local frame, pixel (coalesced along pixels)
       const int frame = f < kGT ? I0 + f : J0 + (f - kGT);
       const long long p = k0 + q;
-      float2 v = make_float2(0.0f, 0.0f);
-      if (frame < n_t && p < k_end) v = iq[(long long)frame * n_pix + p];
-      tile[q][f] = v;
+      pre[u] = (frame < n_t && p < k_end) ? iq[(long long)frame * n_pix + p] : make_float2(0.0f, 0.0f);
     }
     if (tid < kGK) {
       const long long p = k0 + tid;
-      wk[tid] = p < k_end ? (mask ? 1.0 / (double)mask[p] : 1.0) : 0.0;
+      wpre = p < k_end ? (mask ? 1.0 / (double)mask[p] : 1.0) : 0.0;
     }
+  };
+  if (k_begin < k_end) load_slice(k_begin);
+  for (long long k0 = k_begin; k0 < k_end; k0 += kGK) {
+    __syncthreads();   // the previous slice's MFMAs have read the tile
+#pragma unroll
+    for (int u = 0; u < kSlice; ++u) {
+      const int e = tid + 256 * u;
+      tile[e % kGK][e / kGK] = pre[u];
+    }
+    if (tid < kGK) wk[tid] = wpre;
     __syncthreads();
+    if (k0 + kGK < k_end) load_slice(k0 + kGK);
 #pragma unroll 4
     for (int ks = 0; ks < kGK; ks += 4) {
       const int kk = ks + (lane >> 4);
