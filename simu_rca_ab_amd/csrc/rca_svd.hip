@@ -52,7 +52,9 @@ int fail(int code, const char* fmt, ...) {
 constexpr long long kChunkPix = 1LL << 20;   // pixels per projection GEMM
 constexpr int kGT = 32;                      // Gram tile: frames per side
 constexpr int kGK = 64;                      // Gram: pixels per staged slice
-constexpr int kGRow = 2 * kGT + 16;          // LDS row (one pixel): 64 frames + pad
+constexpr int kGRow = 2 * kGT + 1;           // LDS row (one pixel): 64 frames + 1; the odd
+                                             // stride keeps the staging stores (64 pixels of one
+                                             // frame per wave) free of bank conflicts
 constexpr int kMaxDevices = 64;
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
