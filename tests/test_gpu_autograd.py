@@ -55,3 +55,34 @@ def test_differentiable_beamforming_training_runs(dev):
     y.abs().pow(2).sum().backward()
     assert x.grad is not None and torch.isfinite(torch.view_as_real(x.grad)).all()
     assert x.grad.abs().max() > 0
+
+
+def test_adjoint_central_finite_differences(dev):
+    """The loss L(x) = sum |DAS(x) - t|^2 is quadratic in the I/Q, so the
+    central difference (L(x + h v) - L(x - h v)) / 2h with h = 1e-3 equals the
+    directional derivative Re<grad, v> up to FP32 rounding of the forward
+    (relative 1e-3 here); grad comes from the adjoint kernel through
+    torch.autograd (conjugate-Wirtinger convention: dL = Re sum conj(grad) v)."""
+    cfg = C.scaled(C.get("cfg0"), nx=6, ny=5, nz=16)
+    say("case adjoint vs central finite differences, h=1e-3, 3 random directions")
+    plan, G, T = plan_for(cfg, 0, dev, "lanczos3")
+    x = H.iq_frame(cfg, 0, 2).to(dev)
+    gen = torch.Generator(device=dev).manual_seed(11)
+    tgt = torch.randn(2, 1, plan.n_pix, dtype=torch.complex64, device=dev, generator=gen)
+
+    def loss(z):
+        return (das(z, plan, True) - tgt).to(torch.complex128).abs().pow(2).sum()
+
+    xg = x.clone().requires_grad_()
+    loss(xg).backward()
+    g = xg.grad.to(torch.complex128)
+    h = 1e-3
+    scale = x.abs().max()
+    for k in range(3):
+        v = torch.randn(x.shape, dtype=torch.complex64, device=dev, generator=gen) * scale
+        with torch.no_grad():
+            fd = (loss(x + h * v) - loss(x - h * v)).item() / (2 * h)
+        an = (g.conj() * v.to(torch.complex128)).real.sum().item()
+        r = abs(fd - an) / abs(an)
+        say(f"direction {k}: finite difference {fd:.6e}, adjoint {an:.6e}, rel {r:.2e}")
+        assert r <= 1e-3
