@@ -374,3 +374,30 @@ def test_plan_from_host_arrays(dev):
         check(out.cpu().numpy(), ref.sum(axis=1, keepdims=True), "C plan vs oracle")
     finally:
         _lib.check(lib.rca_plan_destroy(handle))
+
+
+def test_offsets_beyond_int32(dev):
+    """260 frames of one config-1 half: 1.09e9 complex samples, so the
+    reference's int32 float index 2 (c Ns + t Ns nCh + f Ns nCh nTx)
+    (bfVirtualSourceGeneric_RCA.cu:315-317) would overflow in the last frames.
+    Frames 0 and 259 carry the same synthetic I/Q, the others zeros: both
+    volumes equal the single-frame result and an empty frame stays zero."""
+    cfg = C.get("cfg1")
+    nf = 260
+    say(f"case cfg1 half 0 x {nf} frames (8.7 GB of I/Q, offsets past 2^31 floats)")
+    one = synth.frame(cfg)[0].to(dev)                     # [1, T, C, Ns]
+    nt, nc, ns = one.shape[1:]
+    assert 2 * nf * nt * nc * ns > 2**31
+    big = torch.zeros((nf, nt, nc, ns), dtype=torch.complex64, device=dev)
+    big[0] = one[0]
+    big[nf - 1] = one[0]
+    plan, _, _ = plan_for(cfg, 0, dev)
+    ref, _ = plan.forward(one, want_mask=False)
+    out, _ = plan.forward(big, want_mask=False)
+    torch.cuda.synchronize()
+    for f in (0, nf - 1):
+        r = H.rel_l2(out[f].cpu().numpy(), ref[0].cpu().numpy())
+        say(f"frame {f}: rel-L2 vs the single-frame launch {r:.3e}")
+        assert r <= 1e-6
+    assert torch.count_nonzero(out[nf // 2]).item() == 0
+    del big, out
