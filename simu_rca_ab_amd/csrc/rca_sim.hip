@@ -69,7 +69,7 @@ int hip_fail(hipError_t e, const char* what) {
 
 constexpr int kThreads = 256;
 #ifndef RCA_SIM_F
-#define RCA_SIM_F 4   // frequencies per lane: 4 measured best (2: -3 %, 8: -21 %)
+#define RCA_SIM_F 6   // frequencies per lane: 6 measured best (4: +3 % time, 2: +6 %, 8: +21 %)
 #endif
 constexpr int kF = RCA_SIM_F;
 constexpr float kPiRef = 3.14159265359f;         // the kernels' float PI (SimUS_CUDARCA.cu:8)
