@@ -1606,14 +1606,18 @@ int bfVirtualSourceGeneric_RCA(float* dataBf, float* mask, const float* data,
 // ---------------------------------------------------------------------------
 // Batched launches for the serving loop (persistent, graph-capturable).
 // ---------------------------------------------------------------------------
-// LDS of a batched launch: the forward budget, or one tilt of whole-trace
-// rows when that is larger (FP32 V = kBatchV32, FB = 1; FP16 V = 1,
-// FB = kBatchFB16; 16-byte rows).
-static size_t batch_lds(int n_sample_max) {
+// LDS of a batched launch per sample type: the standalone kernels' budget
+// (32 KB for FP32 single-frame blocks, 48 KB for FP16 frame blocks), or one
+// tilt of whole-trace rows when that is larger (FP32 V = kBatchV32, FB = 1;
+// FP16 V = 1, FB = kBatchFB16; 16-byte rows).  The FP32 budget matters: at
+// 48 KB only 3 workgroups fit a CU (3 waves per SIMD instead of 4).
+static size_t batch_lds(int n_sample_max, int iq_dtype) {
   const size_t ns = ((size_t)n_sample_max + 3) / 4 * 4;
-  const size_t f32 = (size_t)kBatchV32 * kThreads * sizeof(TxTerm) + ns * 8;
-  const size_t f16 = (size_t)kThreads * sizeof(TxTerm) + (size_t)kBatchFB16 * ns * 4;
-  size_t need = f32 > f16 ? f32 : f16;
+  if (iq_dtype == RCA_IQ_C64) {
+    const size_t need = (size_t)kBatchV32 * kThreads * sizeof(TxTerm) + ns * 8;
+    return need > kFwdLds1 ? need : kFwdLds1;
+  }
+  const size_t need = (size_t)kThreads * sizeof(TxTerm) + (size_t)kBatchFB16 * ns * 4;
   return need > kFwdLds ? need : kFwdLds;
 }
 
@@ -1682,7 +1686,7 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
       const size_t sb = E.iq_dtype == RCA_IQ_C64 ? 8 : 4;
       a.vec_ok = (((uintptr_t)E.iq[h] & 15) == 0) && (((size_t)d->n_sample * sb) % 16 == 0);
       const FwdLayout L = fwd_layout(a, E.iq_dtype == RCA_IQ_C64 ? kBatchV32 : 1, fb, sb,
-                                     tc_max(fb, (int)sb), batch_lds(n_sample_max));
+                                     tc_max(fb, (int)sb), batch_lds(n_sample_max, E.iq_dtype));
       a.tc = L.tc;
       a.wcap = L.wcap;
       args[na].k = a;
@@ -1709,8 +1713,9 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int max_items, int inte
   g_err.clear();
   if (!dev_ws || max_halves < 1 || max_items < 1 || n_sample_max < 1)
     return fail(RCA_ERR_INVALID, "bad batch launch arguments");
-  const size_t lds = batch_lds(n_sample_max);
-  if (lds > (size_t)kMaxLds) return fail(RCA_ERR_UNSUPPORTED, "nSample %d exceeds LDS", n_sample_max);
+  const size_t lds32 = batch_lds(n_sample_max, RCA_IQ_C64), lds16 = batch_lds(n_sample_max, RCA_IQ_F16X2);
+  if (lds32 > (size_t)kMaxLds || lds16 > (size_t)kMaxLds)
+    return fail(RCA_ERR_UNSUPPORTED, "nSample %d exceeds LDS", n_sample_max);
   const unsigned char* ws = reinterpret_cast<const unsigned char*>(dev_ws);
   const int* hdr = reinterpret_cast<const int*>(ws);
   const BatchArgs* args = reinterpret_cast<const BatchArgs*>(ws + batch_args_off());
@@ -1720,9 +1725,9 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int max_items, int inte
 #define RCA_BATCH_CASE(K)                                                                       \
     case K:                                                                                     \
       for (int h = 0; h < 2; ++h)                                                               \
-        hipLaunchKernelGGL((das_fwd_batch<K, float2>), grid, block, lds, stream, hdr, args, items, h); \
+        hipLaunchKernelGGL((das_fwd_batch<K, float2>), grid, block, lds32, stream, hdr, args, items, h); \
       for (int h = 0; h < 2; ++h)                                                               \
-        hipLaunchKernelGGL((das_fwd_batch<K, __half2>), grid, block, lds, stream, hdr, args, items, h); \
+        hipLaunchKernelGGL((das_fwd_batch<K, __half2>), grid, block, lds16, stream, hdr, args, items, h); \
       break;
     RCA_BATCH_CASE(RCA_INTERP_QUADRATIC)
     RCA_BATCH_CASE(RCA_INTERP_LANCZOS_REF)
