@@ -134,3 +134,26 @@ def test_backend_failure_reaches_waiters():
     with pytest.raises(RuntimeError, match="server failed"):
         r.wait(5)
     srv.close(5)
+
+
+def test_gpu_backend_chunks_start_on_fp16_frame_blocks():
+    """GpuBackend.max_frames_per_chunk (host logic only): FP16 chunks are a
+    multiple of the batched kernel's FP16 frame block, at least one block,
+    FP32 chunks any size >= 1; both within the quantum and the work-list
+    capacity when a frame fits them."""
+    from types import SimpleNamespace
+    from simu_rca_ab_amd.serving import GpuBackend
+    for fb16 in (2, 4):
+        be = SimpleNamespace(n_elem=128, max_items=1 << 17, fb16=fb16)
+        for nxy, nt, nf in ((64, 8, 100), (100, 16, 40), (160, 64, 7), (128, 32, 8)):
+            for dt in ("fp32", "fp16"):
+                sp = RequestSpec(nxy, nxy, 2 * nxy, nt, nf, dt)
+                q = int(1.5e10)
+                f = GpuBackend.max_frames_per_chunk(be, sp, q)
+                per = sp.updates(be.n_elem) // sp.n_frame
+                assert f >= 1
+                if dt == "fp16":
+                    assert f % fb16 == 0 and f >= fb16
+                if per <= q and dt == "fp32":
+                    assert f * per <= q
+                assert f * -(-sp.n_vox // 256) <= be.max_items or f <= fb16
