@@ -68,10 +68,20 @@ int hip_fail(hipError_t e, const char* what) {
 }
 
 constexpr int kThreads = 256;
-#ifndef RCA_SIM_F
-#define RCA_SIM_F 6   // frequencies per lane: 6 measured best (4: +3 % time, 2: +6 %, 8: +21 %)
+// Frequencies per lane.  The kernels are instantiated per elevation model
+// (flat sinc or focused four-Gaussian term) and baffle, so each carries only
+// its own path's registers, and the block size is chosen per model
+// (measured): flat 6 (about 100 VGPRs, 5 waves per SIMD; 8 is 1 % slower),
+// focused 2 (about 140 VGPRs; the four-Gaussian term holds ~47 VGPRs per
+// frequency, so 4 frequencies take ~240).
+#ifndef RCA_SIM_F_FLAT
+#define RCA_SIM_F_FLAT 6
 #endif
-constexpr int kF = RCA_SIM_F;
+#ifndef RCA_SIM_F_FOCUSED
+#define RCA_SIM_F_FOCUSED 2
+#endif
+template <bool FLAT>
+constexpr int block_f() { return FLAT ? RCA_SIM_F_FLAT : RCA_SIM_F_FOCUSED; }
 constexpr float kPiRef = 3.14159265359f;         // the kernels' float PI (SimUS_CUDARCA.cu:8)
 constexpr double kInv2Pi = 0.15915494309189533577;
 
@@ -268,15 +278,17 @@ Tr make_tr(const float* ele, int n_ele, int n_sub, float width, float height, fl
 }
 
 // Per-lane frequency constants of a wave's block.
+template <int KF>
 struct FBlock {
-  float kr[kF], ki[kF];
-  double cyc[kF];   // Re(k)/(2 pi): cycles per metre
+  float kr[KF], ki[KF];
+  double cyc[KF];   // Re(k)/(2 pi): cycles per metre
 };
 
-__device__ __forceinline__ FBlock load_block(const float2* k, int f0, int n_f) {
-  FBlock b;
+template <int KF>
+__device__ __forceinline__ FBlock<KF> load_block(const float2* k, int f0, int n_f) {
+  FBlock<KF> b;
 #pragma unroll
-  for (int i = 0; i < kF; ++i) {
+  for (int i = 0; i < KF; ++i) {
     const float2 kk = k[min(f0 + i, n_f - 1)];
     b.kr[i] = kk.x;
     b.ki[i] = kk.y;
@@ -285,17 +297,19 @@ __device__ __forceinline__ FBlock load_block(const float2* k, int f0, int n_f) {
   return b;
 }
 
-// Sum over one element's sub-elements, for the block's frequencies.
-__device__ __forceinline__ void element_sum(const Tr& T, const FBlock& B, int e, double sx,
-                                            double sy, double sz, float2 (&h)[kF]) {
+// Sum over one element's sub-elements, for the block's frequencies (FLAT and
+// BAFFLE are the transducer's T.flat and T.baffle, fixed per instantiation).
+template <bool FLAT, bool BAFFLE, int KF>
+__device__ __forceinline__ void element_sum(const Tr& T, const FBlock<KF>& B, int e, double sx,
+                                            double sy, double sz, float2 (&h)[KF]) {
 #pragma unroll
-  for (int i = 0; i < kF; ++i) h[i] = make_float2(0.0f, 0.0f);
+  for (int i = 0; i < KF; ++i) h[i] = make_float2(0.0f, 0.0f);
   for (int u = 0; u < T.n_sub; ++u) {
     const float* p = T.ele + e + (long long)u * 5 * T.n_ele;
     const Geo g = geometry(sx, sy, sz, p, T.n_ele);
 #pragma unroll
-    for (int i = 0; i < kF; ++i)
-      h[i] = cadd(h[i], sub_term(g, B.kr[i], B.ki[i], B.cyc[i], T.hw, T.baffle, T.flat, T.inv_h2,
+    for (int i = 0; i < KF; ++i)
+      h[i] = cadd(h[i], sub_term(g, B.kr[i], B.ki[i], B.cyc[i], T.hw, BAFFLE, FLAT, T.inv_h2,
                                  T.rfocus));
   }
 }
@@ -312,7 +326,9 @@ struct TxArgs {
   int n_scat, n_f, n_tx;
 };
 
+template <bool FLAT, bool BAFFLE>
 __global__ void __launch_bounds__(kThreads) sim_tx(const TxArgs a) {
+  constexpr int kF = block_f<FLAT>();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int f0 = (blockIdx.y * 4 + wave) * kF;
@@ -322,14 +338,14 @@ __global__ void __launch_bounds__(kThreads) sim_tx(const TxArgs a) {
   const int sc = min(s, a.n_scat - 1);
   const float* rs = a.r_scat + (long long)t * 3 * a.n_scat;
   const double sx = rs[sc], sy = rs[sc + a.n_scat], sz = rs[sc + 2 * a.n_scat];
-  const FBlock B = load_block(a.k, f0, a.n_f);
+  const FBlock<kF> B = load_block<kF>(a.k, f0, a.n_f);
   const int ne = a.T.n_ele;
   float2 acc[kF];
 #pragma unroll
   for (int i = 0; i < kF; ++i) acc[i] = make_float2(0.0f, 0.0f);
   for (int e = 0; e < ne; ++e) {
     float2 h[kF];
-    element_sum(a.T, B, e, sx, sy, sz, h);
+    element_sum<FLAT, BAFFLE>(a.T, B, e, sx, sy, sz, h);
     // transmit wave front Apod exp(j w Delay) (.cu:281-285), w = Re(k) c
     const float ap = a.apod[e + t * ne];
     const double dl = (double)a.delay[e + t * ne] * a.T.c;
@@ -365,7 +381,9 @@ struct RxfArgs {
   int n_scat, n_f;
 };
 
+template <bool FLAT, bool BAFFLE>
 __global__ void __launch_bounds__(kThreads) sim_rxfield(const RxfArgs a) {
+  constexpr int kF = block_f<FLAT>();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int f0 = (blockIdx.y * 4 + wave) * kF;
@@ -375,9 +393,9 @@ __global__ void __launch_bounds__(kThreads) sim_rxfield(const RxfArgs a) {
   const int e = blockIdx.x * 64 + lane;
   const int ec = min(e, ne - 1);
   const double sx = a.r_scat[s], sy = a.r_scat[s + a.n_scat], sz = a.r_scat[s + 2 * a.n_scat];
-  const FBlock B = load_block(a.k, f0, a.n_f);
+  const FBlock<kF> B = load_block<kF>(a.k, f0, a.n_f);
   float2 h[kF];
-  element_sum(a.T, B, ec, sx, sy, sz, h);
+  element_sum<FLAT, BAFFLE>(a.T, B, ec, sx, sy, sz, h);
   if (e >= ne) return;
   const double lc = a.T.lens * a.T.c;
 #pragma unroll
@@ -402,7 +420,9 @@ struct RxArgs {
   int n_scat, n_f, n_tx, nfft;
 };
 
+template <bool FLAT, bool BAFFLE>
 __global__ void __launch_bounds__(kThreads) sim_rx(const RxArgs a) {
+  constexpr int kF = block_f<FLAT>();
   // the four waves of a workgroup share one frequency block and take every
   // fourth scatterer; their sums meet in LDS (fixed order)
   __shared__ float2 s_part[3][kF][64];
@@ -413,7 +433,7 @@ __global__ void __launch_bounds__(kThreads) sim_rx(const RxArgs a) {
   const int ne = a.T.n_ele;
   const int e = blockIdx.x * 64 + lane;
   const int ec = min(e, ne - 1);
-  const FBlock B = load_block(a.k, f0, a.n_f);
+  const FBlock<kF> B = load_block<kF>(a.k, f0, a.n_f);
   const float* rs = a.r_scat + (long long)t * 3 * a.n_scat;
   const float2* txt = a.tx + (long long)t * a.n_scat * a.n_f;
   float2 acc[kF];
@@ -430,7 +450,7 @@ __global__ void __launch_bounds__(kThreads) sim_rx(const RxArgs a) {
     if (!any) continue;   // wave-uniform: a scatterer with no transmit field adds nothing
     const double sx = rs[s], sy = rs[s + a.n_scat], sz = rs[s + 2 * a.n_scat];
     float2 h[kF];
-    element_sum(a.T, B, ec, sx, sy, sz, h);
+    element_sum<FLAT, BAFFLE>(a.T, B, ec, sx, sy, sz, h);
 #pragma unroll
     for (int i = 0; i < kF; ++i) acc[i] = cadd(acc[i], cmul(tv[i], h[i]));
   }
@@ -494,11 +514,42 @@ __global__ void sim_raw(const float2* spec, long long n_batch, int nfft, int n_o
 std::mutex g_fft_mu;
 std::map<std::tuple<int, int, int>, hipfftHandle> g_plans;
 
-dim3 field_grid(int lanes, int n_f, int z) {
-  const int per_block = 4 * kF;
+dim3 field_grid(int lanes, int n_f, int z, int kf) {
+  const int per_block = 4 * kf;
   return dim3((unsigned)((lanes + 63) / 64), (unsigned)((n_f + per_block - 1) / per_block),
               (unsigned)z);
 }
+
+// Launch the instantiation of a field kernel for the transducer's elevation
+// model and baffle.
+template <bool F, bool B>
+void launch_tx(const TxArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL((sim_tx<F, B>), field_grid(a.n_scat, a.n_f, a.n_tx, block_f<F>()), dim3(kThreads),
+                     0, st, a);
+}
+template <bool F, bool B>
+void launch_rxfield(const RxfArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL((sim_rxfield<F, B>), field_grid(a.T.n_ele, a.n_f, a.n_scat, block_f<F>()),
+                     dim3(kThreads), 0, st, a);
+}
+template <bool F, bool B>
+void launch_rx(const RxArgs& a, hipStream_t st) {
+  constexpr int kf = block_f<F>();
+  hipLaunchKernelGGL((sim_rx<F, B>),
+                     dim3((unsigned)((a.T.n_ele + 63) / 64), (unsigned)((a.n_f + kf - 1) / kf),
+                          (unsigned)a.n_tx),
+                     dim3(kThreads), 0, st, a);
+}
+#define RCA_SIM_DISPATCH(fn, T, ...)                                    \
+  do {                                                                  \
+    if ((T).flat) {                                                     \
+      if ((T).baffle) fn<true, true>(__VA_ARGS__);                      \
+      else fn<true, false>(__VA_ARGS__);                                \
+    } else {                                                            \
+      if ((T).baffle) fn<false, true>(__VA_ARGS__);                     \
+      else fn<false, false>(__VA_ARGS__);                               \
+    }                                                                   \
+  } while (0)
 
 int check_tr(const float* ele, int n_ele, int n_sub, float height, float c) {
   if (!ele) return fail(RCA_ERR_INVALID, "element positions are NULL");
@@ -538,7 +589,7 @@ int rca_sim_txfield(void* TX, const float* r_scat, const float* r_ele, const voi
   a.n_scat = NscatPerEvent;
   a.n_f = Nf;
   a.n_tx = Ntx;
-  hipLaunchKernelGGL(sim_tx, field_grid(NscatPerEvent, Nf, Ntx), dim3(kThreads), 0, stream, a);
+  RCA_SIM_DISPATCH(launch_tx, a.T, a, stream);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "sim_tx launch");
 }
@@ -562,7 +613,7 @@ int rca_sim_rxfield(void* RX, const float* r_scat, const float* r_ele, const voi
   a.ir = reinterpret_cast<const float2*>(IR);
   a.n_scat = Nscat;
   a.n_f = Nf;
-  hipLaunchKernelGGL(sim_rxfield, field_grid(Nele, Nf, Nscat), dim3(kThreads), 0, stream, a);
+  RCA_SIM_DISPATCH(launch_rxfield, a.T, a, stream);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "sim_rxfield launch");
 }
@@ -606,11 +657,7 @@ int rca_sim_receive(const rca_sim_receive_args* p, hipStream_t stream) {
       hipLaunchKernelGGL(sim_zero_tail, dim3(1024), dim3(256), 0, stream, a.spec_ab, rows, p->n_f,
                          p->nfft);
   }
-  if (p->n_f > 0)
-    hipLaunchKernelGGL(sim_rx,
-                       dim3((unsigned)((p->n_ele + 63) / 64), (unsigned)((p->n_f + kF - 1) / kF),
-                            (unsigned)p->n_tx),
-                       dim3(kThreads), 0, stream, a);
+  if (p->n_f > 0) RCA_SIM_DISPATCH(launch_rx, a.T, a, stream);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "sim_rx launch");
 }
