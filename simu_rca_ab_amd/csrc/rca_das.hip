@@ -79,6 +79,11 @@ constexpr int kMaxLds = 64 * 1024;
 
 // Per-launch arguments (kernarg).  Geometry pointers come from rca_das_desc;
 // the FP64 scalars are derived on the host from the FP32 parameters.
+// Aberration-law instantiations of the forward kernel.
+constexpr int kLawLine = 0;     // no law, or one delay per receive line
+constexpr int kLawTxLine = 1;   // one delay per (tilt, line)
+constexpr int kLawPixel = 2;    // receive and transmit delays per (pixel, line)
+
 struct KArgs {
   const float *gx, *gy, *gz, *ex, *ey, *ez, *vx, *vy, *vz, *d0, *ab;
   const float* abt;    // PIXEL_CHANNEL transmit-side law (ab is the receive side)
@@ -609,11 +614,14 @@ __device__ __forceinline__ bool line_window(const KArgs& a, const TileBox& b, do
 //
 // CMP: compound over tilts (acc per voxel and frame).  Otherwise TC = 1 and
 // accT holds the current tilt's sums over the lines, or one line's terms with
-// nosum.  PX: the PIXEL_CHANNEL law (a separate instantiation, so the other
-// kernels keep their register budget).  NS: NoSum with tilt chunks - every
-// update is stored as it is computed (the output was cleared before the
-// launch), so a chunk of TC tilts shares each line's receive terms.
-template <int KIND, int V, int FB, int TC, bool CMP, typename S, bool PX = false, bool NS = false>
+// nosum.  LAW: kLawLine for no law or the per-line law (a scalar per line),
+// kLawTxLine for the per-(tilt, line) law, kLawPixel for the per-(pixel, line)
+// laws - separate instantiations, so the common kernels carry neither law's
+// registers (the per-(tilt, line) path alone held 7-10 VGPRs, which took the
+// frame-block kernels past 128).  NS: NoSum with tilt chunks - every update is
+// stored as it is computed (the output was cleared before the launch), so a
+// chunk of TC tilts shares each line's receive terms.
+template <int KIND, int V, int FB, int TC, bool CMP, typename S, int LAW = kLawLine, bool NS = false>
 __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned by,
                                           unsigned char* smem_raw) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
@@ -645,8 +653,8 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   const int step = max((wcap - NT + 1) / E * E, E);
   const long long tstride = (long long)a.n_ch * ns;
   const bool rx = a.receive_x != 0;
-  const bool ab_tc = a.ab_mode == RCA_ABLAW_TX_CHANNEL;
-  constexpr bool ab_px = PX;
+  constexpr bool ab_tc = LAW == kLawTxLine;
+  constexpr bool ab_px = LAW == kLawPixel;
   bool ok;   // scratch for voxel_of at the stores
 
   // receive-axis and transmit-axis coordinates, depth, aperture half-width
@@ -791,7 +799,8 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
               const int base = m - LO;
               if ((k > 0 && base < cb) || (!last && base >= cb + step)) continue;
               float q[NT];
-              interp_weights<KIND, FB == 1 && !PX && !NS>(d, q);
+              // (not in NoSum: its single- and multi-frame launches stay bitwise equal)
+              interp_weights<KIND, FB == 1 && LAW != kLawTxLine && !NS>(d, q);
               float pr = er * x.et.x - ei * x.et.y;
               float pi = er * x.et.y + ei * x.et.x;
               if (ab_tc) {
@@ -922,10 +931,10 @@ __host__ __device__ constexpr int tc_max(int fb, int sbytes) {
   return r > RCA_TC_MAX ? RCA_TC_MAX : (r < 1 ? 1 : r);
 }
 
-template <int KIND, int V, int FB, bool CMP, typename S, bool PX, bool NS>
+template <int KIND, int V, int FB, bool CMP, typename S, int LAW, bool NS>
 __global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  fwd_block<KIND, V, FB, (CMP || NS) ? tc_max(FB, sizeof(S)) : 1, CMP, S, PX, NS>(
+  fwd_block<KIND, V, FB, (CMP || NS) ? tc_max(FB, sizeof(S)) : 1, CMP, S, LAW, NS>(
       a, blockIdx.x, blockIdx.y, smem_raw);
 }
 
@@ -1335,7 +1344,7 @@ constexpr size_t kFwdLds = 48 * 1024, kFwdLds1 = 32 * 1024;
 
 template <typename S> struct SName;
 
-template <int KIND, int V, int FB, bool CMP, typename S, bool PX = false, bool NS = false>
+template <int KIND, int V, int FB, bool CMP, typename S, int LAW = kLawLine, bool NS = false>
 int launch_fwd_t(KArgs a, hipStream_t st) {
   const FwdLayout L = fwd_layout(a, V, FB, sizeof(S), (CMP || NS) ? tc_max(FB, sizeof(S)) : 1,
                                  lds_budget(FB == 1 ? kFwdLds1 : kFwdLds));
@@ -1346,24 +1355,37 @@ int launch_fwd_t(KArgs a, hipStream_t st) {
   const unsigned gx = (unsigned)fwd_blocks(a.n_pix, a.depth_run, V);
   const unsigned gy = (unsigned)((a.n_frame + FB - 1) / FB);
   if (gx == 0 || gy == 0) return RCA_OK;
-  hipLaunchKernelGGL((das_fwd<KIND, V, FB, CMP, S, PX, NS>), dim3(gx, gy, 1), dim3(kThreads), L.lds, st,
+  hipLaunchKernelGGL((das_fwd<KIND, V, FB, CMP, S, LAW, NS>), dim3(gx, gy, 1), dim3(kThreads), L.lds, st,
                      a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "das_fwd launch");
 }
 
 // Instantiated shapes: compounding (V, FB) in (1|2, 1), (1, 2|4|8); per-tilt
-// and per-line outputs (1, 1|4); with the PIXEL_CHANNEL law (1, 1|4).
+// and per-line outputs (1, 1|4); with the PIXEL_CHANNEL law (1, 1|4) and the
+// TX_CHANNEL law (1, 1|4), NoSum included.
 template <int KIND, typename S>
 int launch_fwd_k(const KArgs& a, int V, int FB, hipStream_t st) {
   if (a.ab_mode == RCA_ABLAW_PIXEL_CHANNEL) {
     if (V != 1 || (FB != 1 && FB != 4))
       return fail(RCA_ERR_UNSUPPORTED, "no pixel-law kernel for V=%d FB=%d", V, FB);
     if (a.compound)
-      return FB == 1 ? launch_fwd_t<KIND, 1, 1, true, S, true>(a, st)
-                     : launch_fwd_t<KIND, 1, 4, true, S, true>(a, st);
-    return FB == 1 ? launch_fwd_t<KIND, 1, 1, false, S, true>(a, st)
-                   : launch_fwd_t<KIND, 1, 4, false, S, true>(a, st);
+      return FB == 1 ? launch_fwd_t<KIND, 1, 1, true, S, kLawPixel>(a, st)
+                     : launch_fwd_t<KIND, 1, 4, true, S, kLawPixel>(a, st);
+    return FB == 1 ? launch_fwd_t<KIND, 1, 1, false, S, kLawPixel>(a, st)
+                   : launch_fwd_t<KIND, 1, 4, false, S, kLawPixel>(a, st);
+  }
+  if (a.ab_mode == RCA_ABLAW_TX_CHANNEL) {
+    if (V != 1 || (FB != 1 && FB != 4))
+      return fail(RCA_ERR_UNSUPPORTED, "no per-(tilt, line) law kernel for V=%d FB=%d", V, FB);
+    if (a.compound)
+      return FB == 1 ? launch_fwd_t<KIND, 1, 1, true, S, kLawTxLine>(a, st)
+                     : launch_fwd_t<KIND, 1, 4, true, S, kLawTxLine>(a, st);
+    if (a.nosum)
+      return FB == 1 ? launch_fwd_t<KIND, 1, 1, false, S, kLawTxLine, true>(a, st)
+                     : launch_fwd_t<KIND, 1, 4, false, S, kLawTxLine, true>(a, st);
+    return FB == 1 ? launch_fwd_t<KIND, 1, 1, false, S, kLawTxLine>(a, st)
+                   : launch_fwd_t<KIND, 1, 4, false, S, kLawTxLine>(a, st);
   }
   if (a.compound) {
 #define RCA_FWD_CASE(VV, FF) \
@@ -1371,8 +1393,8 @@ int launch_fwd_k(const KArgs& a, int V, int FB, hipStream_t st) {
     RCA_FWD_CASE(1, 1) RCA_FWD_CASE(2, 1) RCA_FWD_CASE(1, 2) RCA_FWD_CASE(1, 4) RCA_FWD_CASE(1, 8)
 #undef RCA_FWD_CASE
   } else if (a.nosum) {   // NoSum with tilt chunks (output cleared by the caller)
-    if (V == 1 && FB == 1) return launch_fwd_t<KIND, 1, 1, false, S, false, true>(a, st);
-    if (V == 1 && FB == 4) return launch_fwd_t<KIND, 1, 4, false, S, false, true>(a, st);
+    if (V == 1 && FB == 1) return launch_fwd_t<KIND, 1, 1, false, S, kLawLine, true>(a, st);
+    if (V == 1 && FB == 4) return launch_fwd_t<KIND, 1, 4, false, S, kLawLine, true>(a, st);
   } else {
 #define RCA_FWD_CASE(FF) \
     if (V == 1 && FB == FF) return launch_fwd_t<KIND, 1, FF, false, S>(a, st);
@@ -1394,7 +1416,7 @@ int launch_fwd_s(const KArgs& a, int interp, int V, int FB, hipStream_t st) {
 }
 
 int pick_fb(int n_frame, int n_sample, size_t bytes, bool tiled) {
-  int fb = bytes == 8 ? 4 : 8;   // measured best: FP32 4 frames, FP16 8 frames
+  int fb = 8;   // measured best for FP32 and FP16 compounding (frame-block kernels at <= 128 VGPRs)
   // depth tiles stage short windows, so any FB fits; whole traces must fit
   while (fb > 1 && (fb > n_frame || (!tiled && (size_t)fb * n_sample * bytes +
                                       kThreads * sizeof(TxTerm) > (size_t)kMaxLds))) fb >>= 1;
@@ -1522,8 +1544,8 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
   if (!a.compound && fb > 1) fb = fb >= 4 ? 4 : 1;
   while (fb > 1 && !trace_fits(fb, v)) fb >>= 1;    // a tuned FB that does not fit
   if (!trace_fits(fb, v)) v = 1;
-  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) {   // the instantiated pixel-law shapes
-    v = 1;
+  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL || d->ab_law_mode == RCA_ABLAW_TX_CHANNEL) {
+    v = 1;                                           // the instantiated law shapes
     fb = fb >= 4 ? 4 : 1;
   }
   if (a.nosum && !a.accumulate && d->ab_law_mode != RCA_ABLAW_PIXEL_CHANNEL) {
@@ -1664,8 +1686,8 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
       const rca_das_desc* d = E.half[h];
       int rc = check_desc(d);
       if (rc) return rc;
-      if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL)
-        return fail(RCA_ERR_UNSUPPORTED, "entry %d: per-(pixel, channel) laws are not batched", q);
+      if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL || d->ab_law_mode == RCA_ABLAW_TX_CHANNEL)
+        return fail(RCA_ERR_UNSUPPORTED, "entry %d: per-(pixel, line) and per-(tilt, line) laws are not batched", q);
       const int dr = d->depth_run >= 64 ? d->depth_run : 0;
       if (n_pix >= 0 && (d->n_pix != n_pix || dr != depth_run))
         return fail(RCA_ERR_INVALID, "entry %d: halves differ in nPix or depth_run", q);
