@@ -931,23 +931,22 @@ __host__ __device__ constexpr int tc_max(int fb, int sbytes) {
   return r > RCA_TC_MAX ? RCA_TC_MAX : (r < 1 ? 1 : r);
 }
 
+// FP32 single-frame compounding kernels (the drop-in's, with or without a
+// law) are held to 102 VGPRs, 5 waves per SIMD instead of 4: the compiler
+// spills a few values to scratch (outside the per-update loop) and the
+// kernels run 8-9 % (no law, every interpolation kind) and 5 % (laws)
+// faster, measured.  The FP16, frame-block and NoSum kernels ran slower that
+// way and keep the default.
+template <int V, int FB, bool CMP, typename S>
+constexpr int fwd_waves() { return (V == 1 && FB == 1 && CMP && sizeof(S) == 8) ? 5 : 1; }
+
 template <int KIND, int V, int FB, bool CMP, typename S, int LAW, bool NS>
-__global__ void __launch_bounds__(kThreads) das_fwd(const KArgs a) {
+__global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(fwd_waves<V, FB, CMP, S>())))
+das_fwd(const KArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   fwd_block<KIND, V, FB, (CMP || NS) ? tc_max(FB, sizeof(S)) : 1, CMP, S, LAW, NS>(
       a, blockIdx.x, blockIdx.y, smem_raw);
-}
-
-// The FP32 single-frame compounding kernel (the drop-in's) held to 102 VGPRs,
-// 5 waves per SIMD instead of 4: the compiler spills a few values (about 12
-// VGPRs, outside the per-update loop) and the kernel runs 8 % faster,
-// measured.  The FP16 and frame-block kernels ran slower that way.
-template <int KIND>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5)))
-das_fwd_w5(const KArgs a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  fwd_block<KIND, 1, 1, tc_max(1, 8), true, float2, kLawLine, false>(a, blockIdx.x, blockIdx.y,
-                                                                     smem_raw);
 }
 
 // ----------------------------------------------------------------------------
@@ -1367,11 +1366,8 @@ int launch_fwd_t(KArgs a, hipStream_t st) {
   const unsigned gx = (unsigned)fwd_blocks(a.n_pix, a.depth_run, V);
   const unsigned gy = (unsigned)((a.n_frame + FB - 1) / FB);
   if (gx == 0 || gy == 0) return RCA_OK;
-  if constexpr (sizeof(S) == 8 && V == 1 && FB == 1 && CMP && LAW == kLawLine && !NS)
-    hipLaunchKernelGGL((das_fwd_w5<KIND>), dim3(gx, gy, 1), dim3(kThreads), L.lds, st, a);
-  else
-    hipLaunchKernelGGL((das_fwd<KIND, V, FB, CMP, S, LAW, NS>), dim3(gx, gy, 1), dim3(kThreads), L.lds,
-                       st, a);
+  hipLaunchKernelGGL((das_fwd<KIND, V, FB, CMP, S, LAW, NS>), dim3(gx, gy, 1), dim3(kThreads), L.lds, st,
+                     a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "das_fwd launch");
 }
