@@ -10,10 +10,14 @@
  *
  * Plain C: device pointers, sizes and scalars.  The Gram matrix is a
  * hand-written FP64 matrix-core kernel; the filter also calls rocSOLVER
- * (ZHEEVD) and rocBLAS (ZGEMM, the projection), which keep a handle and an
- * internal workspace per device; the caller's workspace holds the FP64 copy
- * of the data.  The call returns when the filter is done (it synchronizes the
- * stream to read the eigensolver's status).
+ * (ZHEEVD), which keeps a handle and an internal workspace per device.  When
+ * the kept or the removed set of eigenvectors has at most 16 members (a
+ * clutter filter removing a few tissue vectors), the projection is one
+ * hand-written FP64 pass over the complex64 data, Y - (Y U) U^H or (Y U) U^H
+ * with U the smaller set; otherwise it is a rocBLAS ZGEMM by the nT x nT
+ * projector on an FP64 copy of the data in the caller's workspace.  The
+ * call returns when the filter is done (it synchronizes the stream to read
+ * the eigensolver's status).
  */
 #ifndef RCA_SVD_H
 #define RCA_SVD_H

@@ -19,7 +19,12 @@
 //   ZGEMM           Y P, in chunks of pixels (rocBLAS)
 //   to_c64          complex64 output in the input layout (hand-written)
 // Projecting with P = V_k V_k^H equals (Y V_k) V_k^H and does not depend on
-// the phase of each eigenvector.
+// the phase of each eigenvector.  When the kept or the removed set has at
+// most kLowRank vectors (a clutter filter removes a few tissue vectors), the
+// last four steps become one pass over the complex64 data instead:
+//   gather_cols     U = the nr columns of the smaller set (hand-written)
+//   project_lowrank Y U U^H, or Y - Y U U^H for the removed set (V is
+//                   unitary), one lane per pixel in FP64 (hand-written)
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -56,6 +61,7 @@ constexpr int kGRow = 2 * kGT + 1;           // LDS row (one pixel): 64 frames +
                                              // stride keeps the staging stores (64 pixels of one
                                              // frame per wave) free of bank conflicts
 constexpr int kMaxDevices = 64;
+constexpr int kLowRank = 16;                  // largest set projected without ZGEMM
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
@@ -241,6 +247,81 @@ __global__ void projector(const double2* __restrict__ V, const unsigned char* __
   P[i + (long long)j * n] = make_double2(re, im);
 }
 
+// Columns of the low-rank projection: U[s * nr + r] = V[s + col[r] * n].
+struct LowRankCols {
+  int col[kLowRank];
+};
+
+__global__ void gather_cols(const double2* __restrict__ V, const LowRankCols cols, int n, int nr,
+                            double2* __restrict__ U) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  for (int r = 0; r < nr; ++r) U[(long long)s * nr + r] = V[s + (long long)cols.col[r] * n];
+}
+
+// Low-rank projection, one lane per pixel, FP64.  With c_r = sum_s Y[s][p]
+// U[s][r] over the nr (<= NR) columns U of the smaller set:
+//   kept set (sgn = +1, base 0):     out[t][p] = sum_r c_r conj(U[t][r])
+//   removed set (sgn = -1, base Y):  out[t][p] = Y[t][p] - sum_r c_r conj(U[t][r])
+// which are Y V_k V_k^H either way (V V^H = I).  Y = iq / sqrt(mask) is read
+// from the complex64 data twice (coalesced over pixels; U is the same for
+// every lane), so the kernel moves 3 x 8 bytes per sample: HBM-bound.
+template <int NR>
+__global__ void __launch_bounds__(256) project_lowrank(const float2* __restrict__ iq,
+                                                       const float* __restrict__ mask,
+                                                       long long n_pix, int n_t,
+                                                       const double2* __restrict__ U, int nr,
+                                                       int removed, float2* __restrict__ out) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pix) return;
+  const double wt = mask ? 1.0 / sqrt((double)mask[p]) : 1.0;
+  double cr[NR], ci[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) cr[r] = ci[r] = 0.0;
+  const float2* x = iq + p;
+  for (int t = 0; t < n_t; ++t) {
+    const float2 v = x[(long long)t * n_pix];
+    const double yr = v.x * wt, yi = v.y * wt;
+    const double2* u = U + (long long)t * nr;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r < nr) {
+        const double2 b = u[r];
+        cr[r] = fma(yr, b.x, fma(-yi, b.y, cr[r]));
+        ci[r] = fma(yr, b.y, fma(yi, b.x, ci[r]));
+      }
+    }
+  }
+  const double sgn = removed ? -1.0 : 1.0;
+  float2* o = out + p;
+  for (int t = 0; t < n_t; ++t) {
+    double zr = 0.0, zi = 0.0;
+    if (removed) {
+      const float2 v = x[(long long)t * n_pix];
+      zr = v.x * wt;
+      zi = v.y * wt;
+    }
+    const double2* u = U + (long long)t * nr;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r < nr) {
+        const double2 b = u[r];   // c_r conj(b)
+        zr = fma(sgn, fma(cr[r], b.x, ci[r] * b.y), zr);
+        zi = fma(sgn, fma(ci[r], b.x, -cr[r] * b.y), zi);
+      }
+    }
+    o[(long long)t * n_pix] = make_float2((float)zr, (float)zi);
+  }
+}
+
+template <int NR>
+hipError_t launch_lowrank(const float2* iq, const float* mask, long long n_pix, int n_t,
+                          const double2* U, int nr, int removed, float2* out, hipStream_t st) {
+  hipLaunchKernelGGL(project_lowrank<NR>, dim3((unsigned)((n_pix + 255) / 256)), dim3(256), 0, st,
+                     iq, mask, n_pix, n_t, U, nr, removed, out);
+  return hipGetLastError();
+}
+
 __global__ void eig_descending(const double* __restrict__ w, int n, double* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] = w[n - 1 - k];
@@ -313,12 +394,6 @@ int rca_svd_filter(const void* iq, const float* mask, long long n_pix, int n_t,
   double2* CH = reinterpret_cast<double2*>(ws + L.chunk);
   const int ni = (int)n_pix;
 
-  const unsigned gx = (unsigned)((n_pix + 255) / 256 < 1024 ? (n_pix + 255) / 256 : 1024);
-  hipLaunchKernelGGL(weigh_to_c128, dim3(gx, n_t), dim3(256), 0, stream,
-                     static_cast<const float2*>(iq), mask, n_pix, Y);
-  if ((he = hipGetLastError()) != hipSuccess)
-    return fail(RCA_ERR_HIP, "weigh_to_c128: %s", hipGetErrorString(he));
-
   double2* PART = reinterpret_cast<double2*>(ws + L.part);
   hipLaunchKernelGGL(gram_f64, dim3(L.npairs, L.splits), dim3(256), 0, stream,
                      static_cast<const float2*>(iq), mask, n_pix, n_t, L.nb, L.per_split, PART);
@@ -333,25 +408,56 @@ int rca_svd_filter(const void* iq, const float* mask, long long n_pix, int n_t,
                         reinterpret_cast<rocblas_double_complex*>(G), n_t, W, E, info);
   if (st != rocblas_status_success) return blas_fail(st, "rocsolver_zheevd");
 
-  if ((he = hipMemcpyAsync(keep_d, keep, (size_t)n_t, hipMemcpyHostToDevice, stream)) != hipSuccess)
-    return fail(RCA_ERR_HIP, "keep upload: %s", hipGetErrorString(he));
-  hipLaunchKernelGGL(projector, dim3((n_t + 63) / 64, n_t), dim3(64), 0, stream, G, keep_d, n_t, P);
-  if ((he = hipGetLastError()) != hipSuccess)
-    return fail(RCA_ERR_HIP, "projector: %s", hipGetErrorString(he));
-
-  const rocblas_double_complex c1(1.0, 0.0), c0(0.0, 0.0);
-  for (long long p0 = 0; p0 < n_pix; p0 += L.chunk_pix) {
-    const long long np = n_pix - p0 < L.chunk_pix ? n_pix - p0 : L.chunk_pix;
-    st = rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, (int)np, n_t, n_t, &c1,
-                       reinterpret_cast<const rocblas_double_complex*>(Y + p0), ni,
-                       reinterpret_cast<const rocblas_double_complex*>(P), n_t, &c0,
-                       reinterpret_cast<rocblas_double_complex*>(CH), (int)np);
-    if (st != rocblas_status_success) return blas_fail(st, "rocblas_zgemm");
-    const unsigned cx = (unsigned)((np + 255) / 256 < 1024 ? (np + 255) / 256 : 1024);
-    hipLaunchKernelGGL(to_c64, dim3(cx, n_t), dim3(256), 0, stream, CH, np, p0, np, n_pix,
-                       static_cast<float2*>(out));
+  // the smaller of the kept and removed sets, as eigenvector columns (the
+  // eigensolver's order is ascending: keep[k] is column n_t - 1 - k)
+  int n_keep = 0;
+  for (int k = 0; k < n_t; ++k) n_keep += keep[k] != 0;
+  const int removed = n_keep > n_t - n_keep;
+  const int nr = removed ? n_t - n_keep : n_keep;
+  if (nr <= kLowRank) {
+    LowRankCols cols = {};
+    int r = 0;
+    for (int k = 0; k < n_t; ++k)
+      if ((keep[k] != 0) != (removed != 0)) cols.col[r++] = n_t - 1 - k;
+    double2* U = P;   // n_t x nr <= n_t x n_t
+    if (nr > 0) {
+      hipLaunchKernelGGL(gather_cols, dim3((n_t + 255) / 256), dim3(256), 0, stream, G, cols, n_t,
+                         nr, U);
+      if ((he = hipGetLastError()) != hipSuccess)
+        return fail(RCA_ERR_HIP, "gather_cols: %s", hipGetErrorString(he));
+    }
+    const float2* x = static_cast<const float2*>(iq);
+    float2* o = static_cast<float2*>(out);
+    he = nr <= 4   ? launch_lowrank<4>(x, mask, n_pix, n_t, U, nr, removed, o, stream)
+         : nr <= 8 ? launch_lowrank<8>(x, mask, n_pix, n_t, U, nr, removed, o, stream)
+                   : launch_lowrank<16>(x, mask, n_pix, n_t, U, nr, removed, o, stream);
+    if (he != hipSuccess) return fail(RCA_ERR_HIP, "project_lowrank: %s", hipGetErrorString(he));
+  } else {
+    const unsigned gx = (unsigned)((n_pix + 255) / 256 < 1024 ? (n_pix + 255) / 256 : 1024);
+    hipLaunchKernelGGL(weigh_to_c128, dim3(gx, n_t), dim3(256), 0, stream,
+                       static_cast<const float2*>(iq), mask, n_pix, Y);
     if ((he = hipGetLastError()) != hipSuccess)
-      return fail(RCA_ERR_HIP, "to_c64: %s", hipGetErrorString(he));
+      return fail(RCA_ERR_HIP, "weigh_to_c128: %s", hipGetErrorString(he));
+    if ((he = hipMemcpyAsync(keep_d, keep, (size_t)n_t, hipMemcpyHostToDevice, stream)) != hipSuccess)
+      return fail(RCA_ERR_HIP, "keep upload: %s", hipGetErrorString(he));
+    hipLaunchKernelGGL(projector, dim3((n_t + 63) / 64, n_t), dim3(64), 0, stream, G, keep_d, n_t, P);
+    if ((he = hipGetLastError()) != hipSuccess)
+      return fail(RCA_ERR_HIP, "projector: %s", hipGetErrorString(he));
+
+    const rocblas_double_complex c1(1.0, 0.0), c0(0.0, 0.0);
+    for (long long p0 = 0; p0 < n_pix; p0 += L.chunk_pix) {
+      const long long np = n_pix - p0 < L.chunk_pix ? n_pix - p0 : L.chunk_pix;
+      st = rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, (int)np, n_t, n_t, &c1,
+                         reinterpret_cast<const rocblas_double_complex*>(Y + p0), ni,
+                         reinterpret_cast<const rocblas_double_complex*>(P), n_t, &c0,
+                         reinterpret_cast<rocblas_double_complex*>(CH), (int)np);
+      if (st != rocblas_status_success) return blas_fail(st, "rocblas_zgemm");
+      const unsigned cx = (unsigned)((np + 255) / 256 < 1024 ? (np + 255) / 256 : 1024);
+      hipLaunchKernelGGL(to_c64, dim3(cx, n_t), dim3(256), 0, stream, CH, np, p0, np, n_pix,
+                         static_cast<float2*>(out));
+      if ((he = hipGetLastError()) != hipSuccess)
+        return fail(RCA_ERR_HIP, "to_c64: %s", hipGetErrorString(he));
+    }
   }
   if (eigval) {
     hipLaunchKernelGGL(eig_descending, dim3((n_t + 255) / 256), dim3(256), 0, stream, W, n_t, eigval);

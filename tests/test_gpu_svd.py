@@ -8,6 +8,9 @@ rel-L2 <= 1e-6 on the filtered ensemble (complex64 rounding is 6e-8) and
 band and stop bands, both reference conventions (Python and MATLAB, with a
 noise mask), a multi-chunk projection (> 2^20 pixels), torch and numpy
 inputs, and the beamformer's own FP16 ensemble output filtered in place.
+Both projection paths run: the low-rank pass (kept or removed set of at most
+16 vectors: high3, low2, stop, none, all) and the ZGEMM by the full
+projector (band, and the multi-chunk case).
 """
 import numpy as np
 import pytest
@@ -29,7 +32,8 @@ def frames_of(Y):
 
 
 @pytest.mark.parametrize("npix,nt,band", [(20000, 64, "high3"), (12000, 200, "band"),
-                                          (5000, 32, "stop")])
+                                          (5000, 32, "stop"), (8000, 48, "low2"),
+                                          (3000, 24, "none"), (3000, 24, "all")])
 def test_filter_vs_oracle(dev, npix, nt, band):
     say(f"case svd filter npix={npix} nt={nt} keep={band}")
     Y, _, _ = S.tissue_blood_ensemble(npix, nt, rank=3, tissue_db=40.0, seed=npix)
@@ -40,25 +44,36 @@ def test_filter_vs_oracle(dev, npix, nt, band):
     elif band == "band":
         keep[:] = False
         keep[int(0.1 * nt):int(0.9 * nt)] = True
-    else:
+    elif band == "stop":
         keep[5:nt // 2] = False
+    elif band == "low2":
+        keep[:] = False
+        keep[:2] = True
+    elif band == "none":
+        keep[:] = False
     out, ev = CF.svd_filter(frames_of(Y), keep)
     torch.cuda.synchronize()
     ref, w = S.project(Y, keep)
-    r = H.rel_l2(out.cpu().numpy().T, ref)
+    got = out.cpu().numpy().T
     e = np.max(np.abs(ev.cpu().numpy() - w) / w[0])
-    say(f"rel-L2 {r:.3e}; eigenvalues max rel err {e:.3e}")
-    assert r <= REL
+    if band == "none":
+        say(f"max |out| {np.abs(got).max():.3e}; eigenvalues max rel err {e:.3e}")
+        assert not np.any(got)
+    else:
+        r = H.rel_l2(got, ref)
+        say(f"rel-L2 {r:.3e}; eigenvalues max rel err {e:.3e}")
+        assert r <= REL
     assert e <= 1e-9
 
 
 def test_multi_chunk_projection(dev):
-    npix, nt = (1 << 20) + 4321, 12
-    say(f"case svd filter over {npix} pixels (two projection chunks), nt={nt}")
+    """20 kept and 20 removed vectors: the ZGEMM path, in two pixel chunks."""
+    npix, nt = (1 << 20) + 4321, 40
+    say(f"case svd filter over {npix} pixels (two projection chunks), nt={nt}, keep 10..29")
     Y, _, _ = S.tissue_blood_ensemble(npix, nt, rank=2, tissue_db=30.0, seed=3)
     Y = Y.astype(np.complex64).astype(np.complex128)
-    keep = np.ones(nt, bool)
-    keep[:2] = False
+    keep = np.zeros(nt, bool)
+    keep[10:30] = True
     out, ev = CF.svd_filter(frames_of(Y), keep)
     torch.cuda.synchronize()
     ref, w = S.project(Y, keep)
