@@ -8,8 +8,9 @@ rel-L2 <= 1e-6 on the filtered ensemble (complex64 rounding is 6e-8) and
 band and stop bands, both reference conventions (Python and MATLAB, with a
 noise mask), a multi-chunk projection (> 2^20 pixels), torch and numpy
 inputs, and the beamformer's own FP16 ensemble output filtered in place.
-Both projection paths run: the low-rank pass (kept or removed set of at most
-16 vectors: high3, low2, stop, none, all) and the ZGEMM by the full
+Both projection paths run: the low-rank kernel (kept or removed set of at
+most 16 vectors: high3 over 64 and 300 frames, low2, none, all for its
+4-vector instantiation, high6 for 8, stop for 16) and the ZGEMM by the full
 projector (band, and the multi-chunk case).
 """
 import numpy as np
@@ -33,14 +34,15 @@ def frames_of(Y):
 
 @pytest.mark.parametrize("npix,nt,band", [(20000, 64, "high3"), (12000, 200, "band"),
                                           (5000, 32, "stop"), (8000, 48, "low2"),
-                                          (3000, 24, "none"), (3000, 24, "all")])
+                                          (3000, 24, "none"), (3000, 24, "all"),
+                                          (4000, 300, "high3"), (6000, 40, "high6")])
 def test_filter_vs_oracle(dev, npix, nt, band):
     say(f"case svd filter npix={npix} nt={nt} keep={band}")
     Y, _, _ = S.tissue_blood_ensemble(npix, nt, rank=3, tissue_db=40.0, seed=npix)
     Y = Y.astype(np.complex64).astype(np.complex128)   # the GPU sees complex64
     keep = np.ones(nt, bool)
-    if band == "high3":
-        keep[:3] = False
+    if band.startswith("high"):
+        keep[:int(band[4:])] = False
     elif band == "band":
         keep[:] = False
         keep[int(0.1 * nt):int(0.9 * nt)] = True
