@@ -137,8 +137,9 @@ __host__ __device__ inline void tile_pair(int pr, int nb, int& bi, int& bj) {
 
 // One workgroup: the 32 x 32 complex block G[I0.., J0..] of Y^H Y over the
 // pixels [k_begin, k_end) of its split, in FP64 on the matrix cores.
-// Real MFMAs on the complex product conj(a) b:
-//   re += a_r b_r + a_i b_i,   im += a_r b_i - a_i b_r.
+// Real MFMAs on the complex product conj(a) b = (a_r b_r + a_i b_i) +
+// i (a_r b_i - a_i b_r), three per step instead of four (Gauss's product:
+// three sums of real products, combined once at the end).
 // The mask weight 1/mask_k (= (1/sqrt(mask_k))^2) scales the b operand.
 // Slices of kGK pixels x (32 + 32) frames are staged in LDS as [pixel][frame],
 // so the 16 lanes that read one pixel's frames read 128 contiguous bytes.
@@ -158,7 +159,9 @@ __global__ void __launch_bounds__(256) gram_f64(const float2* __restrict__ iq,
   const long long k_end = min(n_pix, k_begin + per_split);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wi = w & 1, wj = w >> 1;   // the wave's 16 x 16 sub-block
-  d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+  // conj(a) b with three real products (Gauss): K1 = (a_r - a_i) b_r,
+  // K2 = a_r (b_i - b_r), K3 = -a_i (b_r + b_i); re = K1 - K3, im = K1 + K2
+  d4 k1 = {0.0, 0.0, 0.0, 0.0}, k2 = {0.0, 0.0, 0.0, 0.0}, k3 = {0.0, 0.0, 0.0, 0.0};
   // software pipeline: the next slice's samples (kSlice per thread) and mask
   // weight are loaded into registers while the MFMAs consume the current one
   constexpr int kSlice = 2 * kGT * kGK / 256;
@@ -196,17 +199,16 @@ __global__ void __launch_bounds__(256) gram_f64(const float2* __restrict__ iq,
       const float2 b = tile[kk][kGT + wj * 16 + (lane & 15)];
       const double s = wk[kk];
       const double ar = a.x, ai = a.y, br = (double)b.x * s, bim = (double)b.y * s;
-      accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, accr, 0, 0, 0);
-      accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bim, accr, 0, 0, 0);
-      acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bim, acci, 0, 0, 0);
-      acci = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, br, acci, 0, 0, 0);
+      k1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar - ai, br, k1, 0, 0, 0);
+      k2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bim - br, k2, 0, 0, 0);
+      k3 = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, br + bim, k3, 0, 0, 0);
     }
   }
   double2* dst = partial + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * (kGT * kGT);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int row = wi * 16 + (lane >> 4) + 4 * r, col = wj * 16 + (lane & 15);
-    dst[row * kGT + col] = make_double2(accr[r], acci[r]);
+    dst[row * kGT + col] = make_double2(k1[r] - k3[r], k1[r] + k2[r]);
   }
 }
 
