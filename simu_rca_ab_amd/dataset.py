@@ -83,4 +83,98 @@ def realigned_dataset(RF_X, RF_Y, GeometryX, GeometryY, TransmitX, TransmitY, Pa
     return al, al_c
 
 
-__all__ = ["normalize_rf", "realignment_grids", "realigned_half", "realigned_dataset"]
+# ---------------------------------------------------------------------------
+# On-disk form of one simulation (save_simus_sh.m:396-407) and the per-job
+# arrays the HDF5 conversion builds from it (Python/Convert_job.py:27-84).
+# ---------------------------------------------------------------------------
+MAT_NAMES = ("dataBf_al", "dataBf_al_ab", "dataBf_al_ab_c", "DelaysAb", "Pos", "Pos_bubble")
+POS_MAX_SCATTERERS = 500   # Convert_job.py:42-44: "Pos" rows are padded to 500
+
+
+def delays_ab(delay_x, delay_y, n_az):
+    """``DelaysAb = cat(3, repmat(AbLawX.Delay, 128, 1, naAz),
+    repmat(AbLawY.Delay.', 1, 128, naAz))`` (save_simus_sh.m:307-310): the
+    row-transmit law varies along the second axis, the column-transmit law
+    along the first; [n, n, 2 naAz] float64."""
+    dx = np.asarray(delay_x, dtype=np.float64).ravel()
+    dy = np.asarray(delay_y, dtype=np.float64).ravel()
+    n = dx.size
+    if dy.size != n:
+        raise ValueError("both laws need one delay per element")
+    X = np.broadcast_to(dx[None, :, None], (n, n, n_az))
+    Y = np.broadcast_to(dy[:, None, None], (n, n, n_az))
+    return np.concatenate([X, Y], axis=2)
+
+
+def realigned_set(RF_X, RF_Y, RF_X_ab, RF_Y_ab, GeometryX, GeometryY, TransmitX, TransmitY, Param,
+                  Options, pos_bubble, wavelength, AbLawX, AbLawY, device=None):
+    """The three beamformed arrays of one simulation (save_simus_sh.m:349-394):
+    ``dataBf_al`` from the clean RF, ``dataBf_al_ab_c`` from the aberrated RF
+    with the receive law applied, ``dataBf_al_ab`` from the aberrated RF
+    without it; each complex64 [Nz, nTx, 2 nCh, nFrame]."""
+    al = realigned_dataset(RF_X, RF_Y, GeometryX, GeometryY, TransmitX, TransmitY, Param, Options,
+                           pos_bubble, wavelength, device=device)
+    al_ab, al_ab_c = realigned_dataset(RF_X_ab, RF_Y_ab, GeometryX, GeometryY, TransmitX, TransmitY,
+                                       Param, Options, pos_bubble, wavelength, AbLawX, AbLawY,
+                                       device=device)
+    return {"dataBf_al": al, "dataBf_al_ab": al_ab, "dataBf_al_ab_c": al_ab_c}
+
+
+def save_simulation(path, arrays: dict):
+    """Write the per-simulation set of save_simus_sh.m:396-407 into folder
+    ``path``: one MAT-file per variable, named after it, uncompressed
+    (``save(..., '-nocompression')``), readable by ``scipy.io.loadmat`` as
+    Convert_job.py reads it.  ``arrays`` holds the six names of MAT_NAMES
+    (the beamformed arrays complex64, the others float64)."""
+    import os
+    from scipy.io import savemat
+    missing = [n for n in MAT_NAMES if n not in arrays]
+    if missing:
+        raise ValueError(f"missing variables {missing}")
+    os.makedirs(path, exist_ok=True)
+    for name in MAT_NAMES:
+        a = np.asarray(arrays[name])
+        if name.startswith("data"):
+            a = a.astype(np.complex64, copy=False)
+        else:
+            a = a.astype(np.float64, copy=False)
+        savemat(os.path.join(path, name + ".mat"), {name: a}, do_compression=False,
+                oned_as="column")
+
+
+def load_simulation(path) -> dict:
+    """Read a simulation folder as Convert_job.py:13-14, 31 does: variable
+    names are the file names without '.mat', each loaded by name."""
+    import os
+    from scipy.io import loadmat
+    names = sorted(f[:-4] for f in os.listdir(path) if f.endswith(".mat"))
+    return {n: loadmat(os.path.join(path, n + ".mat"))[n] for n in names}
+
+
+def job_arrays(sim_folders) -> dict:
+    """The datasets Convert_job.py:27-84 appends to a job's HDF5 file, as
+    numpy arrays: for every ``data*`` variable float64 ``<name>_r`` and
+    ``<name>_i`` of shape (nSimu,) + shape; ``Pos`` float64
+    (nSimu, 500, 4, 672) with each simulation's scatterers in the first rows
+    and zeros after; the others float64 (nSimu,) + shape.  Writing them needs
+    h5py (not a dependency here)."""
+    sims = [load_simulation(f) for f in sim_folders]
+    out = {}
+    for name in sims[0]:
+        if name.startswith("data"):
+            out[name + "_r"] = np.stack([np.real(s[name]) for s in sims]).astype(np.float64)
+            out[name + "_i"] = np.stack([np.imag(s[name]) for s in sims]).astype(np.float64)
+        elif name == "Pos":
+            shape = (len(sims), POS_MAX_SCATTERERS) + sims[0][name].shape[1:]
+            pos = np.zeros(shape, np.float64)
+            for i, s in enumerate(sims):
+                pos[i, :s[name].shape[0]] = s[name]
+            out[name] = pos
+        else:
+            out[name] = np.stack([s[name] for s in sims]).astype(np.float64)
+    return out
+
+
+__all__ = ["normalize_rf", "realignment_grids", "realigned_half", "realigned_dataset",
+           "realigned_set", "delays_ab", "save_simulation", "load_simulation", "job_arrays",
+           "MAT_NAMES"]

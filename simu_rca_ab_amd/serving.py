@@ -85,7 +85,11 @@ class Request:
         return self.done - self.arrival
 
     def wait(self, timeout=None):
-        self.event.wait(timeout)
+        """The beamformed volume; raises TimeoutError if it is not ready within
+        ``timeout`` seconds and RuntimeError if the request failed (rejected,
+        server closed or failed)."""
+        if not self.event.wait(timeout):
+            raise TimeoutError(f"request {self.rid} not finished after {timeout} s")
         if self.error:
             raise RuntimeError(self.error)
         return self.result
@@ -330,12 +334,20 @@ class RcaServer:
         self.next_slot = 0
         self.downloads = []              # (req, host, event)
         self._stop = False
+        self._closed = False
         self._thread = None
+        self.error = None                # repr of the exception that stopped the scheduler
 
     # -- client side ------------------------------------------------------
     def submit(self, req: Request, arrival: float | None = None):
+        """Queue a request.  Raises RuntimeError once the server is closed or
+        its scheduler has failed: such a request would never run."""
         req.arrival = time.perf_counter() if arrival is None else arrival
         with self.lock:
+            if self.error is not None:
+                raise RuntimeError(f"server failed: {self.error}")
+            if self._closed:
+                raise RuntimeError("server is closed")
             self.queue.append(req)
         return req
 
@@ -345,9 +357,34 @@ class RcaServer:
         return self
 
     def close(self, timeout=None):
+        """Stop accepting requests, let the batches on the GPU finish and
+        deliver what completes, then fail every request still queued or
+        unfinished (their waiters get RuntimeError instead of blocking)."""
+        with self.lock:
+            self._closed = True
         self._stop = True
         if self._thread is not None:
             self._thread.join(timeout)
+            if self._thread.is_alive():
+                return          # still retiring in-flight batches; it fails the rest on exit
+        self._fail_pending("server closed before the request finished")
+
+    def _fail_pending(self, msg):
+        with self.lock:
+            pending = list(self.queue) + list(self.active)
+            self.queue.clear()
+            self.active = []
+            self.in_use = 0
+        for r in pending:
+            if r.event.is_set():
+                continue
+            r.error = msg
+            r.done = time.perf_counter()
+            try:
+                self.be.release(r)
+            except Exception:   # a failed backend may not release; the request is failed anyway
+                pass
+            r.event.set()
 
     def drain(self, timeout=None):
         """Wait until every submitted request has finished."""
@@ -476,13 +513,13 @@ class RcaServer:
                 if self.slots[j] is not None:
                     self._retire(j)
             self._finish_downloads(block=True)
+            self._fail_pending("server closed before the request finished")
         except Exception as exc:  # surface scheduler failures to every waiter
             with self.lock:
-                pending = list(self.queue) + list(self.active)
-            for r in pending:
-                r.error = f"server failed: {exc!r}"
-                r.event.set()
-            raise
+                self.error = repr(exc)
+                self._closed = True
+            self.downloads = []
+            self._fail_pending(f"server failed: {exc!r}")
 
 
 # ----------------------------------------------------------------------------

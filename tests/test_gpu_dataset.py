@@ -85,3 +85,35 @@ def test_realigned_dataset_concatenates_halves(dev):
     np.testing.assert_array_equal(al[:, :, :CFG.n_elem], realigned_half(rfx, GX, TX_, P, Op, grids, device=dev))
     np.testing.assert_array_equal(al[:, :, CFG.n_elem:], realigned_half(rfy, GY, TY_, P, Op, grids, device=dev))
     assert H.rel_l2(al_c, al) > 1e-3     # the law moves the samples
+
+
+def test_realigned_set_saved_and_read_back(dev, tmp_path):
+    """One simulation's set (save_simus_sh.m:349-407) beamformed on the GPU,
+    written as MAT-files and read back with scipy.io.loadmat: equal to the
+    arrays in memory, and dataBf_al_ab / dataBf_al_ab_c are the aberrated RF
+    beamformed without / with the law."""
+    from simu_rca_ab_amd.dataset import (delays_ab, load_simulation, realigned_set,
+                                         save_simulation)
+    say("case realigned set -> MAT-files -> loadmat")
+    n_tx, n_fr = len(CFG.halves[0].theta), 2
+    GX, TX_, P, Op = H.case(CFG, 0)
+    GY, TY_, _, _ = H.case(CFG, 1)
+    Op = dict(Op, interpolation="lanczos3")
+    rf = [_rf(n_fr, n_tx, s) for s in (21, 22, 23, 24)]
+    bub = np.array([[0.5e-3, -1e-3], [0.2e-3, 0.8e-3], [10e-3, 11e-3]])
+    lam = CFG.c / CFG.fd
+    rng = np.random.default_rng(25)
+    lx = (rng.standard_normal(CFG.n_elem) * 10e-9).astype(np.float32)
+    ly = (rng.standard_normal(CFG.n_elem) * 10e-9).astype(np.float32)
+    s = realigned_set(rf[0], rf[1], rf[2], rf[3], GX, GY, TX_, TY_, P, Op, bub, lam, lx, ly, device=dev)
+    ab, ab_c = realigned_dataset(rf[2], rf[3], GX, GY, TX_, TY_, P, Op, bub, lam, lx, ly, device=dev)
+    np.testing.assert_array_equal(s["dataBf_al_ab"], ab)
+    np.testing.assert_array_equal(s["dataBf_al_ab_c"], ab_c)
+    s["DelaysAb"] = delays_ab(lx, ly, n_tx)
+    s["Pos"] = rng.standard_normal((400, 4, 672))
+    s["Pos_bubble"] = np.vstack([bub, np.ones((1, n_fr))])
+    save_simulation(tmp_path / "1", s)
+    back = load_simulation(tmp_path / "1")
+    for k, v in s.items():
+        np.testing.assert_array_equal(back[k], v)
+    assert back["dataBf_al"].shape == (17, n_tx, 2 * CFG.n_elem, n_fr)

@@ -128,12 +128,59 @@ def test_quantum_splits_large_requests_and_interleaves():
 
 
 def test_backend_failure_reaches_waiters():
+    """A failing launch fails every pending request, stops the scheduler
+    without an unhandled thread exception, and later submits are refused."""
     be = FakeBackend(fail_on_batch=0)
     srv = RcaServer(be, budget_bytes=10**9, quantum=10**6).start()
     r = srv.submit(_req(0))
     with pytest.raises(RuntimeError, match="server failed"):
         r.wait(5)
+    srv._thread.join(5)
+    assert not srv._thread.is_alive()
+    assert "injected launch failure" in srv.error
+    with pytest.raises(RuntimeError, match="server failed"):
+        srv.submit(_req(1))
     srv.close(5)
+
+
+def test_wait_timeout_raises():
+    be = FakeBackend(upd_per_s=1e4)           # 3.2 s of simulated GPU time per frame
+    srv = RcaServer(be, budget_bytes=10**9, quantum=10**9).start()
+    r = srv.submit(_req(0, frames=1))
+    with pytest.raises(TimeoutError):
+        r.wait(0.05)
+    srv.close(10)
+
+
+def test_submit_after_close_raises():
+    srv = RcaServer(FakeBackend(), budget_bytes=10**9, quantum=10**6).start()
+    assert srv.submit(_req(0, frames=1)).wait(5) is not None
+    srv.close(5)
+    with pytest.raises(RuntimeError, match="closed"):
+        srv.submit(_req(1))
+
+
+def test_close_fails_queued_and_unfinished_requests():
+    """close() lets in-flight batches finish and fails the rest: every waiter
+    returns (a result or RuntimeError), none blocks."""
+    be = FakeBackend(upd_per_s=2e6)           # 16 ms per frame
+    per_frame = 1000 * 8 * 4
+    budget = 1000 * 16 * 8                    # one 16-frame request at a time
+    srv = RcaServer(be, budget_bytes=budget, quantum=2 * per_frame).start()
+    reqs = [srv.submit(_req(i, frames=16)) for i in range(3)]
+    time.sleep(0.05)
+    srv.close(10)
+    assert not srv._thread.is_alive()
+    failed = 0
+    for r in reqs:
+        assert r.event.is_set()
+        try:
+            r.wait(0)
+        except RuntimeError as e:
+            assert "closed" in str(e)
+            failed += 1
+    assert failed >= 2                        # the queued ones at least
+    assert not srv.queue and not srv.active
 
 
 def test_gpu_backend_chunks_start_on_fp16_frame_blocks():
