@@ -42,11 +42,24 @@
 
 #include "../../include/rca_das.h"
 #include "rca_math.h"
+#include "rca_tile.h"
 
 namespace {
 
 using rca_math::sincos2pi;
 using rca_math::sqrt_nr;
+using rca_tile::AdjLayout;
+using rca_tile::FwdLayout;
+using rca_tile::TileBox;
+using rca_tile::TxTerm;
+using rca_tile::kMaxLds;
+using rca_tile::kThreads;
+using rca_tile::kTileCols;
+using rca_tile::pass_count;
+using rca_tile::pass_step;
+using rca_tile::sample_pos;
+using rca_tile::split_pos;
+using rca_tile::tc_max;
 
 thread_local std::string g_err;
 
@@ -73,9 +86,6 @@ void clear_error() { g_err.clear(); }
 }  // namespace rca_internal
 
 namespace {
-
-constexpr int kThreads = 256;
-constexpr int kMaxLds = 64 * 1024;
 
 // Per-launch arguments (kernarg).  Geometry pointers come from rca_das_desc;
 // the FP64 scalars are derived on the host from the FP32 parameters.
@@ -287,56 +297,16 @@ __device__ __forceinline__ float2 ld_sample(const float2* s, int i) { return s[i
 __device__ __forceinline__ float2 ld_sample(const __half2* s, int i) { return __half22float2(s[i]); }
 
 // ----------------------------------------------------------------------------
-// Voxel tiles.
-//
-// A workgroup has 256 lanes x V voxels.  With a depth-run hint (the grid's
-// Nz, voxel p = iz + Nz*column) it owns a tile of neighbouring depths and
-// columns, so the sample window of its voxels per (tilt, line) is short; each
-// wave holds 64/CW depths of CW neighbouring columns.  Without the hint a
-// workgroup owns 256V consecutive voxels.  The forward kernel stacks its 4
-// waves in depth with CW = kTileCols (16 depths x 16 columns): neighbouring
-// lanes read nearby (often equal) samples, which spreads its LDS reads over
-// the banks.  The adjoint puts one column in each wave (64 depths x 4
-// columns), so the lanes of a wave scatter to distinct samples.
+// Voxel tiles (rca_tile.h).  The forward kernel stacks its 4 waves in depth
+// with CW = kTileCols (16 depths x 16 columns): neighbouring lanes read nearby
+// (often equal) samples, which spreads its LDS reads over the banks.  The
+// adjoint puts one column in each wave (64 depths x 4 columns), so the lanes
+// of a wave scatter to distinct samples.
 // ----------------------------------------------------------------------------
-#ifndef RCA_TILE_COLS
-#define RCA_TILE_COLS 16   // measured best of 4, 8, 16 on cfg1-sized volumes
-#endif
-constexpr int kTileCols = RCA_TILE_COLS;   // forward: columns per wave (waves stacked in depth)
-
-// Tile of waves: CW columns per wave, DW of the 4 waves stacked in depth
-// (4/DW side by side in columns).  Tile = DW*64/CW depths x CW*(4/DW)*V columns.
 template <int CW, int DW>
 __device__ __forceinline__ long long voxel_of(const KArgs& a, unsigned bx, int V, int v, int tid,
                                               bool& valid) {
-  constexpr int WD = 64 / CW, TD = DW * WD, WC = 4 / DW;
-  if (a.depth_run >= 64) {
-    const long long ndc = (a.depth_run + TD - 1) / TD;
-    const long long dc = bx % ndc, cg = bx / ndc;
-    const int l = tid & 63, w = tid >> 6;
-    const long long col = cg * (CW * WC * V) + v * (CW * WC) + (w / DW) * CW + (l % CW);
-    const long long dep = dc * TD + (w % DW) * WD + l / CW;
-    const long long p = dep + (long long)a.depth_run * col;
-    valid = dep < a.depth_run && p < a.n_pix;
-    return p;
-  }
-  const long long p = (long long)bx * (kThreads * V) + (long long)v * kThreads + tid;
-  valid = p < a.n_pix;
-  return p;
-}
-
-template <int CW, int DW>
-__host__ __device__ inline long long tile_blocks(long long n_pix, int depth_run, int V) {
-  constexpr int TD = DW * (64 / CW), TC = CW * (4 / DW);
-  if (depth_run >= 64) {
-    const long long ncol = (n_pix + depth_run - 1) / depth_run;
-    return ((ncol + TC * V - 1) / (TC * V)) * ((depth_run + TD - 1) / TD);
-  }
-  return (n_pix + kThreads * V - 1) / (kThreads * V);
-}
-
-__host__ __device__ inline long long fwd_blocks(long long n_pix, int depth_run, int V) {
-  return tile_blocks<kTileCols, 4>(n_pix, depth_run, V);
+  return rca_tile::voxel_of<CW, DW>(a.depth_run, a.n_pix, bx, V, v, tid, valid);
 }
 
 // Values every lane of the workgroup holds alike, moved to scalar registers.
@@ -362,58 +332,7 @@ __device__ __forceinline__ double lane_val(double x, int j) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// |t| over t in [lo, hi]: smallest and largest.
-__device__ __forceinline__ void abs_range(double lo, double hi, double& amin, double& amax) {
-  amin = (lo <= 0.0 && hi >= 0.0) ? 0.0 : fmin(fabs(lo), fabs(hi));
-  amax = fmax(fabs(lo), fabs(hi));
-}
-
-// Transmit bounds of one tilt over a box (transmit-axis range [l_lo, l_hi],
-// depth range [z_lo, z_hi]): the sample offset Bu = fs*((|P-VS| - d0)/c + t0)
-// grows with the distance, so its extremes come from the extreme distances.
-__device__ __forceinline__ void tx_bounds(const KArgs& a, double l_lo, double l_hi, double z_lo,
-                                          double z_hi, float vl, float vz, double d0, float& bmin,
-                                          float& bmax) {
-  double l0, l1, z0, z1;
-  abs_range(l_lo - (double)vl, l_hi - (double)vl, l0, l1);
-  abs_range(z_lo - (double)vz, z_hi - (double)vz, z0, z1);
-  bmin = (float)fma(a.kfs, sqrt(l0 * l0 + z0 * z0) - d0, a.ut0);
-  bmax = (float)fma(a.kfs, sqrt(l1 * l1 + z1 * z1) - d0, a.ut0);
-}
-
-// Transmit terms of one (voxel, tilt), cached in LDS for a chunk of tilts.
-struct alignas(16) TxTerm {
-  int bi;      // transmit sample offset fs*((|P-VS| - d0)/c + t0) = bi + bf,
-  float bf;    //   bi = floor, bf in [0, 1)
-  float2 et;   // e^{+j 2 pi frac(Bp)}, Bp = fd*((|P-VS| - d0 - 2z)/c + t0)
-};
-
-// x = i + f with i = floor(x) and f = x - i in [0, 1], computed in FP64.
-// Sample positions are summed from such parts in integer + FP32 arithmetic:
-// the fractions are exact to 2^-25 samples each, far below what the weights
-// and the range test resolve.  (Offsets beyond +-2^30 samples are clamped;
-// such updates are out of range anyway.)
-__device__ __forceinline__ void split_pos(double x, int& i, float& f) {
-  const double fl = fmin(fmax(floor(x), -1073741824.0), 1073741824.0);
-  i = (int)fl;
-  f = (float)(x - fl);
-}
-
-// m = floor(u), d = u - m for u = (bi + bf) + (ri + rf) [+ (ai + af)].
-__device__ __forceinline__ void sample_pos(int i, float f, int& m, float& d) {
-  const float fl = floorf(f);
-  m = i + (int)fl;
-  d = f - fl;
-}
-
 // Coordinates of a workgroup's voxels and their bounding box.
-struct TileBox {
-  float lat_lo, lat_hi;   // receive-axis coordinate
-  float txl_lo, txl_hi;   // transmit-axis coordinate
-  float zlo, zhi;         // depth
-  double apmax;           // largest receive aperture half-width z/F
-};
-
 template <int V, int CW, int DW>
 __device__ __forceinline__ TileBox tile_voxels(const KArgs& a, unsigned bx, int tid, bool rx,
                                                double* pl, double* pt, double* pz, double* apF,
@@ -507,8 +426,8 @@ __device__ __forceinline__ void chunk_tx_bounds(const KArgs& a, const TileBox& b
   bu_hi = -INFINITY;
   if (lane < nt) {
     const int t = t0 + lane;
-    tx_bounds(a, b.txl_lo, b.txl_hi, b.zlo, b.zhi, rx ? a.vy[t] : a.vx[t], a.vz[t], a.d0[t], bu_lo,
-              bu_hi);
+    rca_tile::tx_bounds(a.kfs, a.ut0, b.txl_lo, b.txl_hi, b.zlo, b.zhi, rx ? a.vy[t] : a.vx[t],
+                        a.vz[t], a.d0[t], bu_lo, bu_hi);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -571,14 +490,11 @@ __device__ __forceinline__ bool line_window(const KArgs& a, const TileBox& b, do
                                             double ezd, double uabc, float bu_lo, float bu_hi,
                                             bool ab_tc, int t0, int nt, int c, int lane, int& s0,
                                             int& whi, float px_lo = 0.0f, float px_hi = 0.0f) {
-  double A0, A1, Z0, Z1;
-  abs_range((double)b.lat_lo - e_lat, (double)b.lat_hi - e_lat, A0, A1);
-  abs_range((double)b.zlo - ezd, (double)b.zhi - ezd, Z0, Z1);
-  if (!(A0 < b.apmax)) return false;
-  A1 = fmin(A1, b.apmax);
+  float r_lo, r_hi;
+  if (!rca_tile::rx_bounds(a.kfs, b, e_lat, ezd, uabc, r_lo, r_hi)) return false;
   if (!(px_lo <= px_hi)) return false;   // PIXEL_CHANNEL law: no active voxel
-  float r_lo = (float)(a.kfs * sqrt(A0 * A0 + Z0 * Z0) + uabc) + px_lo;
-  float r_hi = (float)(a.kfs * sqrt(A1 * A1 + Z1 * Z1) + uabc) + px_hi;
+  r_lo += px_lo;
+  r_hi += px_hi;
   if (ab_tc) {   // per-tilt delays of the TX_CHANNEL law: their range over the chunk
     float lo = INFINITY, hi = -INFINITY;
     if (lane < nt) lo = hi = (float)(a.fs * a.ab[(long long)(t0 + lane) * a.n_ch + c]);
@@ -590,10 +506,10 @@ __device__ __forceinline__ bool line_window(const KArgs& a, const TileBox& b, do
     r_lo += uni(lo);
     r_hi += uni(hi);
   }
-  const int ns = a.n_sample;
-  const int w_lo = (int)fminf(fmaxf(0.0f, floorf(bu_lo + r_lo) - (float)(LO + 1)), (float)ns);
-  whi = uni((int)fmaxf(fminf((float)ns, floorf(bu_hi + r_hi) + (float)(HI + 2)), 0.0f));
-  s0 = uni((w_lo / E) * E);
+  int w0, w1;
+  rca_tile::window_from(a.n_sample, LO, HI, E, bu_lo + r_lo, bu_hi + r_hi, w0, w1);
+  whi = uni(w1);
+  s0 = uni(w0);
   return true;
 }
 
@@ -648,9 +564,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   const int wcap = a.wcap;
   const int tcn = (CMP || NS) ? a.tc : 1;   // tilts per chunk (<= TC)
   S* tr = reinterpret_cast<S*>(smem_raw + (size_t)tcn * V * kThreads * sizeof(TxTerm));
-  // pass advance; at least one 16-byte chunk so the pass loop always ends
-  // (rows shorter than NT + E only occur with traces too short for any tap)
-  const int step = max((wcap - NT + 1) / E * E, E);
+  const int step = pass_step(wcap, NT, E);   // pass advance
   const long long tstride = (long long)a.n_ch * ns;
   const bool rx = a.receive_x != 0;
   constexpr bool ab_tc = LAW == kLawTxLine;
@@ -722,7 +636,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
       int s0, whi;
       if (line_window<LO, HI, E>(a, box, e_lat, ezd, uabc, bu_lo, bu_hi, ab_tc, t0, nt, c, lane, s0,
                                  whi, px_lo, px_hi)) {
-        const int npass = whi - s0 > wcap ? 1 + (whi - s0 - wcap + step - 1) / step : 1;
+        const int npass = pass_count(s0, whi, wcap, step);
 
         for (int k = 0; k < npass; ++k) {
           const int cb = s0 + k * step;
@@ -797,7 +711,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
               else sample_pos(x.bi + ri, x.bf + rf, m, d);                 // .cu:310-319
               if (!(m >= LO && m < ns - 1 - HI)) continue;                  // .cu:323
               const int base = m - LO;
-              if ((k > 0 && base < cb) || (!last && base >= cb + step)) continue;
+              if (!rca_tile::in_pass(base, k, cb, last, step)) continue;
               float q[NT];
               // (not in NoSum: its single- and multi-frame launches stay bitwise equal)
               interp_weights<KIND, FB == 1 && LAW != kLawTxLine && !NS>(d, q);
@@ -922,23 +836,18 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   __syncthreads();   // s_box and the LDS rows may be reused by the caller's next block
 }
 
-// Most tilts per chunk: rows of TC x FB windows fit 32 KB at 256 samples.
-#ifndef RCA_TC_MAX
-#define RCA_TC_MAX 8
-#endif
-__host__ __device__ constexpr int tc_max(int fb, int sbytes) {
-  const int r = (sbytes == 8 ? 16 : 32) / fb;
-  return r > RCA_TC_MAX ? RCA_TC_MAX : (r < 1 ? 1 : r);
-}
-
 // FP32 single-frame compounding kernels (the drop-in's, with or without a
 // law) are held to 102 VGPRs, 5 waves per SIMD instead of 4: the compiler
 // spills a few values to scratch (outside the per-update loop) and the
 // kernels run 8-9 % (no law, every interpolation kind) and 5 % (laws)
 // faster, measured.  The FP16, frame-block and NoSum kernels ran slower that
-// way and keep the default.
+// way.  The multi-frame compounding kernels run at 4 waves per SIMD (their FB
+// accumulators need 110-128 VGPRs) and are held there: past 128 VGPRs a SIMD
+// holds 3 of them (-16 to -20 %, measured).
 template <int V, int FB, bool CMP, typename S>
-constexpr int fwd_waves() { return (V == 1 && FB == 1 && CMP && sizeof(S) == 8) ? 5 : 1; }
+constexpr int fwd_waves() {
+  return (V == 1 && FB == 1 && CMP && sizeof(S) == 8) ? 5 : (V == 1 && FB >= 2 && CMP) ? 4 : 1;
+}
 
 template <int KIND, int V, int FB, bool CMP, typename S, int LAW, bool NS>
 __global__ void __launch_bounds__(kThreads)
@@ -1057,7 +966,7 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
     else sample_pos(x.bi + ri, x.bf + rf, m, d);                 // .cu:310-319
     if (!(m >= LO && m < ns - 1 - HI)) continue;                  // .cu:323
     const int base = m - LO;
-    if ((k > 0 && base < cb) || (!last && base >= cb + step)) continue;
+    if (!rca_tile::in_pass(base, k, cb, last, step)) continue;
     float q[NT];
     interp_weights<KIND>(d, q);
     float pr = er * x.et.x - ei * x.et.y;
@@ -1139,7 +1048,7 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
   const size_t wrows = (size_t)tcn * FB * 2 * wcap;   // floats per wave
   float* myrows = rows + wave * wrows;
   int* claim = reinterpret_cast<int*>(rows + 4 * wrows) + wave * wcap;
-  const int step = max(wcap - NT + 1, 1);   // pass advance
+  const int step = pass_step(wcap, NT, 1);   // pass advance
   const bool rx = a.receive_x != 0;
   const bool ab_tc = a.ab_mode == RCA_ABLAW_TX_CHANNEL;
   constexpr bool ab_px = PX;   // PIXEL_CHANNEL law: its own instantiation
@@ -1187,7 +1096,7 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
       if (!line_window<LO, HI, 1>(a, box, e_lat, ezd, uabc, bu_lo, bu_hi, ab_tc, t0, nt, c, lane,
                                   s0, whi, px_lo, px_hi))
         continue;
-      const int npass = whi - s0 > wcap ? 1 + (whi - s0 - wcap + step - 1) / step : 1;
+      const int npass = pass_count(s0, whi, wcap, step);
 
       for (int k = 0; k < npass; ++k) {
         const int cb = s0 + k * step;
@@ -1328,42 +1237,21 @@ KArgs make_args(const rca_das_desc* d, int n_frame) {
   return a;
 }
 
-// LDS layout of a forward launch: tilts per chunk (tc) and samples per row
-// (wcap).  Per tilt of a chunk the workgroup holds V x 256 transmit terms and
-// FB rows.  Depth tiles stage short windows (rows of kTileRow samples; longer
-// windows take several passes); otherwise rows hold whole traces (rounded up
-// to 16 bytes).  tc is what fits the budget, at least 1.
-constexpr int kTileRow = kTileCols >= 8 ? 128 : 256;
-struct FwdLayout { int tc, wcap; size_t lds; };
-FwdLayout fwd_layout(const KArgs& a, int V, int fb, size_t sbytes, int tc_kernel, size_t budget,
-                     int tile_row = kTileRow) {
-  const int e = (int)(16 / sbytes);
-  const int full = (a.n_sample + e - 1) / e * e;
-  FwdLayout L;
-  L.wcap = a.depth_run && full > tile_row ? tile_row : full;
-  const size_t per_tilt = (size_t)V * kThreads * sizeof(TxTerm) + (size_t)fb * L.wcap * sbytes;
-  const long long fit = (long long)(budget / per_tilt);
-  L.tc = (int)(fit < 1 ? 1 : fit > tc_kernel ? tc_kernel : fit);
-  L.lds = (size_t)L.tc * per_tilt;
-  return L;
-}
-
-// LDS budget of a forward workgroup (measured on cfg1-sized volumes):
-// single-frame blocks run best at 32 KB (6 tilts per chunk), multi-frame
-// blocks at 48 KB.
-constexpr size_t kFwdLds = 48 * 1024, kFwdLds1 = 32 * 1024;
+using rca_tile::kFwdLds;
+using rca_tile::kFwdLds1;
 
 template <typename S> struct SName;
 
 template <int KIND, int V, int FB, bool CMP, typename S, int LAW = kLawLine, bool NS = false>
 int launch_fwd_t(KArgs a, hipStream_t st) {
-  const FwdLayout L = fwd_layout(a, V, FB, sizeof(S), (CMP || NS) ? tc_max(FB, sizeof(S)) : 1,
-                                 lds_budget(FB == 1 ? kFwdLds1 : kFwdLds));
+  const FwdLayout L = rca_tile::fwd_layout(a.depth_run, a.n_sample, V, FB, sizeof(S),
+                                           (CMP || NS) ? tc_max(FB, sizeof(S)) : 1,
+                                           lds_budget(FB == 1 ? kFwdLds1 : kFwdLds));
   if (L.lds > (size_t)kMaxLds)
     return fail(RCA_ERR_UNSUPPORTED, "FB=%d traces of %d samples do not fit in LDS", FB, a.n_sample);
   a.tc = L.tc;
   a.wcap = L.wcap;
-  const unsigned gx = (unsigned)fwd_blocks(a.n_pix, a.depth_run, V);
+  const unsigned gx = (unsigned)rca_tile::fwd_blocks(a.n_pix, a.depth_run, V);
   const unsigned gy = (unsigned)((a.n_frame + FB - 1) / FB);
   if (gx == 0 || gy == 0) return RCA_OK;
   hipLaunchKernelGGL((das_fwd<KIND, V, FB, CMP, S, LAW, NS>), dim3(gx, gy, 1), dim3(kThreads), L.lds, st,
@@ -1440,36 +1328,15 @@ int pick_v(long long, int, int, int, bool) {
   return 1;
 }
 
-// Adjoint layout: per tilt of a chunk 256 transmit terms and, for each of the
-// 4 waves, FB rows of complex FP32 accumulators (interleaved re, im); then a
-// claim slot per row sample for each wave.  Depth tiles (64 depths x 4
-// columns) use rows of kAdjTileRow samples, other tiles whole traces; rows
-// are capped so one tilt fits 64 KB (longer windows take several passes).
-constexpr int kAdjTileRow = 192;
-constexpr size_t kAdjLds = 48 * 1024;
-
-struct AdjLayout { int tc, wcap; size_t lds; };
-AdjLayout adj_layout(const KArgs& a, int fb, int tc_kernel, size_t budget) {
-  AdjLayout L;
-  const int w0 = a.depth_run && a.n_sample > kAdjTileRow ? kAdjTileRow : a.n_sample;
-  const int wmax = (int)((kMaxLds - kThreads * sizeof(TxTerm)) / (4 * 8 * (size_t)fb + 4 * 4));
-  L.wcap = w0 < wmax ? w0 : wmax;
-  const size_t per_tilt = kThreads * sizeof(TxTerm) + 4 * 8 * (size_t)fb * L.wcap;
-  const size_t claims = 4 * 4 * (size_t)L.wcap;
-  const long long fit = budget > claims ? (long long)((budget - claims) / per_tilt) : 0;
-  L.tc = (int)(fit < 1 ? 1 : fit > tc_kernel ? tc_kernel : fit);
-  L.lds = L.tc * per_tilt + claims;
-  return L;
-}
-
 template <int KIND, int FB, bool PX = false>
 int launch_adj_t(KArgs a, hipStream_t st) {
-  const AdjLayout L = adj_layout(a, FB, tc_max(FB, 8), lds_budget(kAdjLds));
+  const AdjLayout L = rca_tile::adj_layout(a.depth_run, a.n_sample, FB, tc_max(FB, 8),
+                                           lds_budget(rca_tile::kAdjLds));
   if (L.lds > (size_t)kMaxLds || L.wcap < Interp<KIND>::LO + Interp<KIND>::HI + 1)
     return fail(RCA_ERR_UNSUPPORTED, "FB=%d adjoint rows do not fit in LDS", FB);
   a.tc = L.tc;
   a.wcap = L.wcap;
-  const dim3 grid((unsigned)tile_blocks<1, 1>(a.n_pix, a.depth_run, 1),
+  const dim3 grid((unsigned)rca_tile::tile_blocks<1, 1>(a.n_pix, a.depth_run, 1),
                   (unsigned)((a.n_frame + FB - 1) / FB));
   hipLaunchKernelGGL((das_adj<KIND, FB, PX>), grid, dim3(kThreads), L.lds, st, a);
   const hipError_t e = hipGetLastError();
@@ -1718,15 +1585,16 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
       a.nosum = 0;
       const size_t sb = E.iq_dtype == RCA_IQ_C64 ? 8 : 4;
       a.vec_ok = (((uintptr_t)E.iq[h] & 15) == 0) && (((size_t)d->n_sample * sb) % 16 == 0);
-      const FwdLayout L = fwd_layout(a, E.iq_dtype == RCA_IQ_C64 ? kBatchV32 : 1, fb, sb,
-                                     tc_max(fb, (int)sb), batch_lds(n_sample_max, E.iq_dtype));
+      const FwdLayout L = rca_tile::fwd_layout(a.depth_run, a.n_sample,
+                                               E.iq_dtype == RCA_IQ_C64 ? kBatchV32 : 1, fb, sb,
+                                               tc_max(fb, (int)sb), batch_lds(n_sample_max, E.iq_dtype));
       a.tc = L.tc;
       a.wcap = L.wcap;
       args[na].k = a;
       args[na].iq_dtype = E.iq_dtype;
       ++na;
     }
-    const long long nvb = fwd_blocks(n_pix, depth_run, E.iq_dtype == RCA_IQ_C64 ? kBatchV32 : 1);
+    const long long nvb = rca_tile::fwd_blocks(n_pix, depth_run, E.iq_dtype == RCA_IQ_C64 ? kBatchV32 : 1);
     for (int fbk = E.frame_begin / fb; fbk * fb < E.frame_end; ++fbk) {
       for (long long vb = 0; vb < nvb; ++vb) {
         if (ni >= max_items) return fail(RCA_ERR_UNSUPPORTED, "batch exceeds %d items", max_items);

@@ -59,3 +59,65 @@ def iq_frame(cfg, half_index=0, n_frame=1, seed_shift=0):
             iq = synth.add_noise(iq, cfg.snr_db, cfg.seeds[2] + f)
         outs.append(iq)
     return torch.stack(outs).contiguous()
+
+
+def tile_voxels(n_pix, depth_run, bx, kind="forward", tile_cols=16):
+    """Voxel indices of workgroup ``bx`` of a depth-tiled launch (valid ones,
+    lane order), as the kernels' voxel_of assigns them: forward tiles are 16
+    depths x 16 neighbouring columns (4 waves stacked in depth), adjoint tiles
+    64 depths x 4 columns (one column per wave)."""
+    cw, dw = (tile_cols, 4) if kind == "forward" else (1, 1)
+    wd, td, wc = 64 // cw, dw * (64 // cw), 4 // dw
+    ndc = -(-depth_run // td)
+    dc, cg = bx % ndc, bx // ndc
+    tid = np.arange(256)
+    l, w = tid & 63, tid >> 6
+    col = cg * (cw * wc) + (w // dw) * cw + (l % cw)
+    dep = dc * td + (w % dw) * wd + l // cw
+    p = dep + depth_run * col
+    return p[(dep < depth_run) & (p < n_pix)]
+
+
+def n_tiles(n_pix, depth_run, kind="forward", tile_cols=16):
+    """(number of workgroups, depth tiles per column group) of a tiled launch."""
+    cw, dw = (tile_cols, 4) if kind == "forward" else (1, 1)
+    td, tc = dw * (64 // cw), cw * (4 // dw)
+    ncol = -(-n_pix // depth_run)
+    ndc = -(-depth_run // td)
+    return -(-ncol // tc) * ndc, ndc
+
+
+def edge_tiles(nx, ny, nz, n_random=8, seed=0, kind="forward"):
+    """Workgroups of a [Nz, Nx, Ny] grid at the lateral corners and edges and
+    the centre, each at the first, middle and last depth tile, plus a few
+    seeded random ones; returns the union of their voxel indices (sorted) and
+    the workgroup list."""
+    n_pix = nx * ny * nz
+    nblk, ndc = n_tiles(n_pix, nz, kind)
+    tc = 16 if kind == "forward" else 4
+    ngroups = nblk // ndc
+    def grp(ix, iy):
+        return min(ngroups - 1, (iy * nx + ix) // tc)
+    gs = {grp(0, 0), grp(nx - 1, 0), grp(0, ny - 1), grp(nx - 1, ny - 1), grp(nx // 2, 0),
+          grp(0, ny // 2), grp(nx - 1, ny // 2), grp(nx // 2, ny - 1), grp(nx // 2, ny // 2)}
+    blocks = sorted({g * ndc + d for g in gs for d in (0, ndc // 2, ndc - 1)})
+    rng = np.random.default_rng(seed)
+    blocks = sorted(set(blocks) | set(rng.choice(nblk, size=min(n_random, nblk), replace=False).tolist()))
+    vox = np.unique(np.concatenate([tile_voxels(n_pix, nz, b, kind) for b in blocks]))
+    return vox, blocks
+
+
+def rel_l2_t(got, want):
+    """rel-L2 of two complex torch tensors, computed where they are."""
+    import torch
+    d = (got - want).to(torch.complex128)
+    return float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(want.to(torch.complex128)))
+
+
+def envelope_db_err_t(got, want, floor_db=-40.0):
+    """envelope_db_err on torch tensors, computed where they are."""
+    import torch
+    g = got.abs().reshape(-1).to(torch.float64)
+    w = want.abs().reshape(-1).to(torch.float64)
+    keep = w >= w.max() * 10 ** (floor_db / 20)
+    return float((20 * torch.log10(g[keep].clamp_min(1e-30)) - 20 * torch.log10(w[keep])).abs().max())

@@ -1,7 +1,8 @@
 """GPU parity of the depth-tiled forward launch (grid hint ``depth_run``).
 
-With the hint, a workgroup owns 64 depths x 4V columns and stages only the
-sample window that its voxels' taps can reach on each (tilt, line).  Its
+With the hint, a forward workgroup owns 16 depths x 16V neighbouring columns
+(the adjoint 64 depths x 4 columns) and stages only the sample window that its
+voxels' taps can reach on each (tilt chunk, line).  Its
 results must match the untiled launch and the FP64 oracle (1e-5, 0.1 dB).
 Tiled and untiled launches add the same per-voxel terms, but the tilts per
 chunk (set by the LDS budget) fix the order in which tilts are added, so
@@ -28,7 +29,9 @@ pytestmark = pytest.mark.gpu
 
 TOL = 4e-6
 
-TILE = C.scaled(C.get("cfg1"), nx=5, ny=3, nz=128, n_scat=300)   # 15 columns, 2 depth tiles
+# 36 columns: two whole 16-column forward groups and a partial third one that
+# spans two grid rows (9 x 4 columns), 8 forward / 2 adjoint depth tiles
+TILE = C.scaled(C.get("cfg1"), nx=9, ny=4, nz=128, n_scat=300)
 
 
 def plans(cfg, half_index, dev, interp=None, G=None, depth_run=None):
@@ -47,7 +50,7 @@ def plans(cfg, half_index, dev, interp=None, G=None, depth_run=None):
 @pytest.mark.parametrize("interp", ["quadratic", "lanczos", "lanczos3", "lanczos4"])
 @pytest.mark.parametrize("half_index", [0, 1])
 def test_tiled_matches_untiled_and_oracle(dev, interp, half_index):
-    say(f"case tiled 5x3x128 interp={interp} half={half_index}, 3 frames, per tilt + compound")
+    say(f"case tiled 9x4x128 interp={interp} half={half_index}, 3 frames, per tilt + compound")
     iq = H.iq_frame(TILE, half_index, n_frame=3)
     flat, tiled, G = plans(TILE, half_index, dev, interp)
     x = iq.to(dev)
