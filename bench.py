@@ -10,9 +10,12 @@ Metric: voxel*tilt*element updates per second, counted densely
 (nVox * nTilt * nElem per frame).  Prints one JSON line.
 
 Extra legs:
-  * roofline of the dominant kernel (the forward DAS launch): algorithmic bytes
-    = updates per launch x taps x 8 B (the interpolation footprint), divided by
-    the launch's mean duration from HIP events on its stream;
+  * roofline of the dominant kernel (the forward DAS launch), which is bound
+    by vector-ALU issue: VALU lane-instructions per launch (hardware counter
+    profile of this kernel and workload) over the launch's mean duration from
+    HIP events on its stream, against the issue ceiling measured by the FMA
+    probe in the same run; HBM traffic, its fraction of 8 TB/s and the
+    compulsory bytes beside it;
   * cpu_baseline: the FP32 CPU restatement of the reference kernel (OpenMP on
     every host core) on a bounded voxel sample of the same workload;
   * fp16: the FP16-I/Q mode on a 16-frame ensemble of the same geometry.
@@ -33,6 +36,8 @@ sys.path.insert(0, str(ROOT))
 os.environ.setdefault("MIOPEN_FIND_MODE", "FAST")
 
 HBM_PEAK_GBS = 8000.0
+# MI355X FP32 vector peak 157.3 TFLOP/s = 78.6 T lane-FMA/s (2 FLOP per FMA)
+VALU_PEAK_NOMINAL_T = 78.6
 TRAFFIC_PROFILE = ROOT / "profiles" / "das_fwd_cfg1_counters.md"
 
 
@@ -50,6 +55,73 @@ def _profile_value(key):
 def measured_traffic():
     """HBM bytes per das_fwd launch at config 1 (FETCH_SIZE/WRITE_SIZE)."""
     return _profile_value("traffic_bytes_per_launch")
+
+
+def valu_probe(dev, n_blocks=2048, iters=4096, reps=5, mode=0):
+    """Measured VALU issue ceiling in T lane-ops/s: the library's probe of
+    independent instruction chains of one class (mode 0: v_fma_f32), 8 waves
+    per SIMD on 256 CUs, HIP events on its stream."""
+    import ctypes
+    import torch
+    from simu_rca_ab_amd import _lib
+    lib = _lib.load()
+    out = torch.empty(n_blocks * 256, dtype=torch.float32, device=dev)
+    st = torch.cuda.current_stream(dev)
+    lane = ctypes.c_longlong(0)
+    run = lambda: _lib.check(lib.rca_probe_valu(mode, ctypes.c_void_p(out.data_ptr()), n_blocks, iters,
+                                                 ctypes.c_void_p(st.cuda_stream), ctypes.byref(lane)))
+    run()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        run()
+        b.record(st)
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    return lane.value / (statistics.median(ms) * 1e-3) / 1e12, statistics.median(ms)
+
+
+def roofline(cfg, mean_launch_ms, upd_launch, probe_t):
+    """Roofline of the dominant kernel (das_fwd, one config-1 half per launch).
+    The kernel is bound by VALU issue: achieved = VALU lane-instructions per
+    launch (SQ_INSTS_VALU x 64 from the committed counter profile of this
+    kernel on this workload) / the launch's mean duration measured here,
+    against the FMA-chain probe's ceiling measured in the same process.  HBM
+    is reported beside it from the measured traffic (FETCH_SIZE/WRITE_SIZE,
+    copy-calibrated), with the compulsory bytes; the interpolation footprint
+    (taps x 8 B per update) is not HBM traffic, since the windows are staged
+    in LDS: its ratio to the measured traffic is the LDS reuse factor."""
+    t = mean_launch_ms * 1e-3
+    vinst = _profile_value("valu_insts_per_launch")
+    traffic = measured_traffic()
+    achieved = vinst * 64 / t / 1e12 if vinst else None
+    algo = upd_launch * cfg.taps() * 8
+    n_pix = cfg.n_vox
+    iq = len(cfg.halves[0].theta) * cfg.n_elem * cfg.n_sample * 8
+    # per launch, mean of the two halves: I/Q and grid read, output and mask
+    # written (the second half also reads them to accumulate)
+    compulsory = iq + 12 * n_pix + 1.5 * (8 * n_pix + 4 * n_pix)
+    return {"bound": "valu", "achieved": achieved, "peak": probe_t, "unit": "T VALU lane-instr/s",
+            "frac": achieved / probe_t if achieved else None, "traffic": traffic,
+            "peak_source": "measured in this run: rca_probe_valu mode 1 (8 independent v_fma_f32 "
+                           "chains per lane with inline-constant operands, 8 waves/SIMD): one VALU "
+                           "wave-instruction per ~2.2 cycles per SIMD; nominal 78.6 T lane-ops/s = "
+                           "157.3 TFLOP/s FP32 / 2 (2 cycles per wave64 instruction at 2.4 GHz). "
+                           "Instructions with two or three VGPR sources (the tap FMAs) issue at "
+                           "about half that rate (ceiling_two_vgpr_sources, probe mode 0)",
+            "peak_nominal": VALU_PEAK_NOMINAL_T, "frac_of_nominal": achieved / VALU_PEAK_NOMINAL_T if achieved else None,
+            "valu_insts_per_launch": vinst, "valu_lane_instr_per_update": vinst * 64 / upd_launch if vinst else None,
+            "source": "profiles/das_fwd_cfg1_counters.md (SQ_INSTS_VALU, FETCH_SIZE, WRITE_SIZE)",
+            "kernel": "das_fwd<lanczos4, FP32, V=1, FB=1>", "launch_ms": mean_launch_ms,
+            "updates_per_launch": upd_launch,
+            "hbm": {"achieved_gbs": traffic / t / 1e9 if traffic else None, "peak_gbs": HBM_PEAK_GBS,
+                    "frac": traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None,
+                    "compulsory_bytes": compulsory,
+                    "traffic_over_compulsory": traffic / compulsory if traffic else None},
+            "lds_reuse_factor": algo / traffic if traffic else None,
+            "interp_footprint_bytes_per_update": cfg.taps() * 8}
 
 
 def log(*a):
@@ -514,7 +586,14 @@ def main():
                          "Conv3d kernels, which can take minutes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device copy (HBM-counter calibration)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only run the VALU FMA-chain probe (VALU-counter calibration)")
     args = ap.parse_args()
+    if args.valu_probe:
+        import torch
+        t, ms = valu_probe(torch.device("cuda", 0), reps=2)
+        print(json.dumps({"valu_probe_t_lane_fma_per_s": t, "ms": ms}))
+        return
     if args.copy_probe:
         import torch
         n = 512 << 20
@@ -539,12 +618,16 @@ def main():
     value = upd_step / (dt / args.steps)
     mean_launch = statistics.mean(launch_ms)
     upd_launch = upd_step / len(cfg.halves)
-    bytes_launch = upd_launch * cfg.taps() * 8
-    achieved = bytes_launch / (mean_launch * 1e-3) / 1e9
     log(f"[bench] {value / 1e9:.2f} G updates/s, {ms_step:.2f} ms/step, "
         f"launch mean {mean_launch:.2f} ms")
     del plans, iqs, out
     torch.cuda.empty_cache()
+    probe_t, probe_ms = valu_probe(dev, mode=1)
+    probe_t0, _ = valu_probe(dev, mode=0)
+    log(f"[bench] VALU probe: {probe_t:.1f} T lane-ops/s (one VGPR source), {probe_t0:.1f} "
+        f"(two VGPR sources)")
+    roof = roofline(cfg, mean_launch, upd_launch, probe_t)
+    roof["ceiling_two_vgpr_sources"] = probe_t0
     extra = {}
     if not args.no_fp16:
         log("[bench] FP16-I/Q ensemble leg (16 frames)")
@@ -594,15 +677,7 @@ def main():
                                "2048-sample complex64 I/Q, 128x128x256 voxels, Lanczos a=4, 1 frame",
                    "voxels": cfg.n_vox, "tilts": cfg.n_tilt, "elements": cfg.n_elem,
                    "samples": cfg.n_sample, "interp": cfg.interpolation, "frames": 1},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "traffic_source": "profiles/das_fwd_cfg1_counters.md (FETCH_SIZE x 2.00 + "
-                                       "WRITE_SIZE x 1.00, copy-kernel calibrated)",
-                     "kernel": "das_fwd<lanczos4>", "launch_ms": mean_launch,
-                     "valu_busy_pct": _profile_value("valu_busy_pct"),
-                     "binding_unit": "VALU issue (the LDS-staged gather keeps HBM traffic "
-                                     "at 0.96 GB per launch against 550 GB algorithmic)",
-                     "bytes_per_update": cfg.taps() * 8, "updates_per_launch": upd_launch},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -236,6 +236,18 @@ int rca_plan_create(const rca_plan_params* p, rca_plan** plan);
 const rca_das_desc* rca_plan_desc(const rca_plan* plan);
 int rca_plan_destroy(rca_plan* plan);
 
+/*
+ * Measurement probe (no reference counterpart): VALU issue ceilings the DAS
+ * kernels are reported against.  n_blocks workgroups of 256 lanes, each lane
+ * 8 independent chains x 8 x iters instructions of one class: mode 0
+ * v_fma_f32 (VGPR multiplier), 1 v_fma_f32 (inline constants), 2 v_add_f32,
+ * 3 v_mul_f32, 4 v_fma_f64, 5 v_max_f32, 6 v_add_u32.  Writes one float per
+ * lane to out (n_blocks * 256) and the lane-operations of the launch to
+ * *lane_ops.  Asynchronous on stream.
+ */
+int rca_probe_valu(int mode, float* out, int n_blocks, int iters, hipStream_t stream,
+                   long long* lane_ops);
+
 /* Last error message of the calling thread ("" when none). */
 const char* rca_last_error(void);
 

@@ -23,7 +23,7 @@ EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last
            "rca_version", "rca_set_tuning", "rca_batch_workspace_bytes", "rca_batch_encode",
            "rca_batch_launch", "rca_batch_frame_block", "rca_svd_workspace_bytes", "rca_svd_filter", "rca_sim_txfield",
            "rca_sim_rxfield", "rca_sim_receive", "rca_sim_spectra_to_raw", "rca_prepare_geometry",
-           "rca_plan_create", "rca_plan_desc", "rca_plan_destroy")
+           "rca_plan_create", "rca_plan_desc", "rca_plan_destroy", "rca_probe_valu")
 
 
 class RcaDasDesc(ctypes.Structure):
@@ -96,7 +96,8 @@ def load():
     lib.rca_plan_desc.restype = dp
     lib.rca_plan_destroy.argtypes = [vp]
     lib.rca_sim_spectra_to_raw.argtypes = [vp, i, i, i, i, vp, vp]
-    for name in ("rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA",
+    lib.rca_probe_valu.argtypes = [i, vp, i, i, vp, ctypes.POINTER(ll)]
+    for name in ("rca_probe_valu", "rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA",
                  "rca_svd_filter", "rca_sim_txfield", "rca_sim_rxfield", "rca_sim_receive",
                  "rca_sim_spectra_to_raw", "rca_prepare_geometry", "rca_plan_create",
                  "rca_plan_destroy"):

@@ -22,8 +22,9 @@
 //     analytically) before anything is rounded to FP32, so FP32 carries only
 //     quantities in [0,1).  Interpolation weights, the rotation and the
 //     accumulation are FP32.
-//   * Lanczos weights use one sincos per update (trig identities for the
-//     per-tap sin terms) instead of 2a sinpi calls.
+//   * Lanczos weights are evaluated as pairs of even/odd polynomials around
+//     the half-sample point (46 FMA-class instructions for 8 taps) instead of
+//     2a sinpi calls and divides.
 //   * The tilt sum (compounding) happens in registers; one output write per
 //     voxel and frame.
 //   * FP16-I/Q traces (half2 + FP32 per-trace scale) halve LDS and HBM bytes;
@@ -41,6 +42,7 @@
 #include <string>
 
 #include "../../include/rca_das.h"
+#include "rca_interp_poly.h"
 #include "rca_math.h"
 #include "rca_tile.h"
 
@@ -118,28 +120,6 @@ struct KArgs {
 // Device math.
 // ----------------------------------------------------------------------------
 
-// sin t, cos t for |t| <= pi/(2W) (the Lanczos tap identities take
-// t = pi e/W with |e| <= 1/2).  Taylor series truncated where the next term is
-// below 2e-8 at the interval's end: W = 2 (|t| <= pi/4) to t^9 / t^10,
-// W = 3 (<= pi/6) to t^7 / t^8, W = 4 (<= pi/8) to t^7 / t^6.
-template <int W>
-__device__ __forceinline__ void sincos_small(float t, float& s, float& c) {
-  const float t2 = t * t;
-  if constexpr (W <= 2) {
-    s = t * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.75573192e-6f, -1.98412698e-4f), 8.33333333e-3f),
-                          -1.66666667e-1f), 1.0f);
-    c = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.75573192e-7f, 2.48015873e-5f),
-                                        -1.38888889e-3f), 4.16666667e-2f), -0.5f), 1.0f);
-  } else if constexpr (W == 3) {
-    s = t * fmaf(t2, fmaf(t2, fmaf(t2, -1.98412698e-4f, 8.33333333e-3f), -1.66666667e-1f), 1.0f);
-    c = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.48015873e-5f, -1.38888889e-3f), 4.16666667e-2f),
-                      -0.5f), 1.0f);
-  } else {
-    s = t * fmaf(t2, fmaf(t2, fmaf(t2, -1.98412698e-4f, 8.33333333e-3f), -1.66666667e-1f), 1.0f);
-    c = fmaf(t2, fmaf(t2, fmaf(t2, -1.38888889e-3f, 4.16666667e-2f), -0.5f), 1.0f);
-  }
-}
-
 // Hann weight 0.5*(1 + cos(pi*rho)) = cos^2(pi*rho/2) for rho in [0, 1).
 __device__ __forceinline__ float hann(float rho) {
   const float t = rho * 1.57079632679490f;
@@ -154,23 +134,6 @@ __device__ __forceinline__ float hann(float rho) {
   return c * c;
 }
 
-// cos(pi j / W), sin(pi j / W) for the Lanczos tap identities.
-__host__ __device__ constexpr float cs_cos(int W, int j) {
-  const int m = ((j % (2 * W)) + 2 * W) % (2 * W);
-  if (W == 2) return m == 0 ? 1.f : m == 1 ? 0.f : m == 2 ? -1.f : 0.f;
-  if (W == 3) return m == 0 ? 1.f : m == 1 ? 0.5f : m == 2 ? -0.5f : m == 3 ? -1.f : m == 4 ? -0.5f : 0.5f;
-  // W == 4
-  return m == 0 ? 1.f : m == 1 ? 0.70710678118655f : m == 2 ? 0.f : m == 3 ? -0.70710678118655f
-       : m == 4 ? -1.f : m == 5 ? -0.70710678118655f : m == 6 ? 0.f : 0.70710678118655f;
-}
-__host__ __device__ constexpr float cs_sin(int W, int j) {
-  const int m = ((j % (2 * W)) + 2 * W) % (2 * W);
-  if (W == 2) return m == 0 ? 0.f : m == 1 ? 1.f : m == 2 ? 0.f : -1.f;
-  if (W == 3) return m == 0 ? 0.f : m <= 2 ? 0.86602540378444f : m == 3 ? 0.f : -0.86602540378444f;
-  return m == 0 ? 0.f : m == 1 ? 0.70710678118655f : m == 2 ? 1.f : m == 3 ? 0.70710678118655f
-       : m == 4 ? 0.f : m == 5 ? -0.70710678118655f : m == 6 ? -1.f : -0.70710678118655f;
-}
-
 // Interpolation kinds: taps k = -LO..HI around m = floor(u); W = window divisor.
 template <int KIND> struct Interp;
 template <> struct Interp<RCA_INTERP_QUADRATIC> { static constexpr int LO = 1, HI = 1, W = 0; };
@@ -178,24 +141,16 @@ template <> struct Interp<RCA_INTERP_LANCZOS_REF> { static constexpr int LO = 2,
 template <> struct Interp<RCA_INTERP_LANCZOS3> { static constexpr int LO = 2, HI = 3, W = 3; };
 template <> struct Interp<RCA_INTERP_LANCZOS4> { static constexpr int LO = 3, HI = 4, W = 4; };
 
-// Weights q[k+LO] = sinc(d-k) * sinc((d-k)/W), d in [0, 1].
+// Weights q[k+LO] = sinc(d-k) * sinc((d-k)/W), d in [0, 1)
+// (bfVirtualSourceGeneric_RCA.cu:79-82, 331-336 for W = 2).
 //
-// With e = d - r (r = nearest of 0/1, |e| <= 1/2) every tap abscissa is
-// x = e - j for an integer j, so sin(pi x) = (-1)^j sin(pi e) and
-// sin(pi x / W) = sin(pi e/W) cos(pi j/W) - cos(pi e/W) sin(pi j/W):
-// one small-angle sincos serves all taps, and the tap nearest to u (j = 0)
-// is evaluated from e directly (series near 0), which keeps every weight
-// accurate to a few ulp even when d rounds to 0 or 1.
-//
-// The 1/x^2 factors cost a hardware reciprocal each, a quarter-rate
-// instruction.  With BATCH, one reciprocal of their product serves all
-// (prefix products forward, then two multiplies per factor backward: 3(n-1)
-// full-rate multiplies for n-1 reciprocals saved; the factors lie in
-// [1e-12, 20.25], the product well inside FP32 range, relative error a few
-// ulp).  It holds up to eight more VGPRs, so kernels near the 128-VGPR
-// occupancy step (frame blocks, pixel laws, NoSum) keep the per-tap
-// reciprocals.
-template <int KIND, bool BATCH = false>
+// With e = d - 1/2 the taps pair up, q_{1-p}(e) = q_p(-e): the even and odd
+// parts of q_p are polynomials of degree 4 in u = e^2 (rca_interp_poly.h,
+// fitted near-minimax on |e| <= 1/2, max |error| 1.4e-7 in FP32), so a pair
+// of taps costs 10 FMAs and two adds: 46 VALU instructions for the 8 taps of
+// a = 4, against about 95 for the sine-identity form with one reciprocal per
+// update it replaces (same bar against the FP64 oracle).
+template <int KIND>
 __device__ __forceinline__ void interp_weights(float d, float* q) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI;
   if constexpr (KIND == RCA_INTERP_QUADRATIC) {
@@ -204,64 +159,25 @@ __device__ __forceinline__ void interp_weights(float d, float* q) {
     q[1] = -(d - 1.0f) * (d + 1.0f);
     q[2] = d * (d + 1.0f) * 0.5f;
   } else {
-    constexpr int W = Interp<KIND>::W;
-    constexpr float kPi2 = 9.86960440108936f;
-    const bool up = d >= 0.5f;
-    const float e = up ? d - 1.0f : d;
-    float se, ce;
-    sincos_small<W>(e * (3.14159265358979f / (float)W), se, ce);
-    float S;  // sin(pi e) from the multiple-angle identity
-    if constexpr (W == 2) S = 2.0f * se * ce;
-    else if constexpr (W == 3) S = se * fmaf(-4.0f * se, se, 3.0f);
-    else S = 4.0f * se * ce * fmaf(-2.0f * se, se, 1.0f);
-    const float K = S * ((float)W / kPi2);
-    // w_j = (-1)^j K (se cos(pi j/W) - ce sin(pi j/W)) / x^2: the sign and
-    // K go into the two products once, the tap constants into one FMA
-    const float Ks = K * se, Kc = K * ce;
-    constexpr int J0 = -LO - 1;
-    constexpr int NJ = LO + HI + 2;
-    const float e2 = e * e;
-    // weight of tap index i (j = J0 + i) given 1/x^2
-    auto weight = [&](int i, float inv) {
-      const int j = J0 + i;
-      if (j == 0) {
-        constexpr float a2 = kPi2 / 6.0f * (1.0f + 1.0f / (W * W));
-        constexpr float a4 = kPi2 * kPi2 * (1.0f / 120.0f + 1.0f / (120.0f * W * W * W * W) +
-                                            1.0f / (36.0f * W * W));
-        const float ser = fmaf(e2, fmaf(e2, a4, -a2), 1.0f);
-        return fabsf(e) < 1e-3f ? ser : K * se * inv;
-      }
-      const float sg = (j & 1) ? -1.0f : 1.0f;
-      const float cj = sg * cs_cos(W, j), sj = -sg * cs_sin(W, j);
-      const float num = sj == 0.0f ? Ks * cj : cj == 0.0f ? Kc * sj : fmaf(Ks, cj, Kc * sj);
-      return num * inv;
-    };
-    // x^2 of every tap, the centre one clamped away from 0 (below |e| = 1e-3
-    // its weight comes from the series)
-    float x2[NJ], wj[NJ];
+    static_assert(HI == LO + 1 && kPolyDeg == 4, "taps pair as p and 1 - p; degree-4 parts");
+    using P = InterpPoly<KIND>;
+    const float e = d - 0.5f;
+    const float u = e * e;
 #pragma unroll
-    for (int i = 0; i < NJ; ++i) {
-      const float x = e - (float)(J0 + i);
-      x2[i] = J0 + i == 0 ? fmaxf(e2, 1e-12f) : x * x;
+    for (int p = 1; p <= HI; ++p) {
+      const float* A = P::kEven[p - 1];
+      const float* B = P::kOdd[p - 1];
+      float pe = fmaf(u, A[4], A[3]);
+      pe = fmaf(u, pe, A[2]);
+      pe = fmaf(u, pe, A[1]);
+      pe = fmaf(u, pe, A[0]);
+      float po = fmaf(u, B[4], B[3]);
+      po = fmaf(u, po, B[2]);
+      po = fmaf(u, po, B[1]);
+      po = fmaf(u, po, B[0]) * e;
+      q[p + LO] = pe + po;
+      q[1 - p + LO] = pe - po;
     }
-    if constexpr (BATCH) {
-      float pre[NJ];
-      pre[0] = x2[0];
-#pragma unroll
-      for (int i = 1; i < NJ; ++i) pre[i] = pre[i - 1] * x2[i];
-      float r = __builtin_amdgcn_rcpf(pre[NJ - 1]);   // 1 / (x2[0] ... x2[i]) below
-#pragma unroll
-      for (int i = NJ - 1; i > 0; --i) {
-        wj[i] = weight(i, r * pre[i - 1]);
-        r *= x2[i];
-      }
-      wj[0] = weight(0, r);
-    } else {
-#pragma unroll
-      for (int i = 0; i < NJ; ++i) wj[i] = weight(i, __builtin_amdgcn_rcpf(x2[i]));
-    }
-#pragma unroll
-    for (int k = -LO; k <= HI; ++k) q[k + LO] = up ? wj[k - 1 - J0] : wj[k - J0];
   }
 }
 
@@ -713,8 +629,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
               const int base = m - LO;
               if (!rca_tile::in_pass(base, k, cb, last, step)) continue;
               float q[NT];
-              // (not in NoSum: its single- and multi-frame launches stay bitwise equal)
-              interp_weights<KIND, FB == 1 && LAW != kLawTxLine && !NS>(d, q);
+              interp_weights<KIND>(d, q);
               float pr = er * x.et.x - ei * x.et.y;
               float pi = er * x.et.y + ei * x.et.x;
               if (ab_tc) {
@@ -841,13 +756,11 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
 // spills a few values to scratch (outside the per-update loop) and the
 // kernels run 8-9 % (no law, every interpolation kind) and 5 % (laws)
 // faster, measured.  The FP16, frame-block and NoSum kernels ran slower that
-// way.  The multi-frame compounding kernels run at 4 waves per SIMD (their FB
-// accumulators need 110-128 VGPRs) and are held there: past 128 VGPRs a SIMD
-// holds 3 of them (-16 to -20 %, measured).
+// way and keep the default; holding the multi-frame kernels to 4 waves per
+// SIMD (<= 128 VGPRs, which they meet by themselves) cost 9-16 % on the
+// 16-frame ensembles, measured.
 template <int V, int FB, bool CMP, typename S>
-constexpr int fwd_waves() {
-  return (V == 1 && FB == 1 && CMP && sizeof(S) == 8) ? 5 : (V == 1 && FB >= 2 && CMP) ? 4 : 1;
-}
+constexpr int fwd_waves() { return (V == 1 && FB == 1 && CMP && sizeof(S) == 8) ? 5 : 1; }
 
 template <int KIND, int V, int FB, bool CMP, typename S, int LAW, bool NS>
 __global__ void __launch_bounds__(kThreads)

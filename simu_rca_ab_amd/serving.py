@@ -243,16 +243,20 @@ class GpuBackend:
         return iq + scale + spec.n_vox * spec.n_frame * 8 + spec.n_vox * 4
 
     def upload(self, req: Request):
-        """Allocate the request's device buffers and start the H2D copies."""
+        """Allocate the request's device buffers and start the H2D copies
+        (I/Q already on this device, e.g. written there by the simulator, is
+        used in place)."""
         dev = {}
+        # device-resident I/Q may still be in production on the default stream
+        self.copy.wait_stream(torch.cuda.default_stream(self.device))
         with torch.cuda.stream(self.copy):
             dev["iq"], dev["scale"] = [], []
             for h in req.halves:
                 if isinstance(h, tuple):
-                    dev["iq"].append(h[0].to(self.device, non_blocking=True))
-                    dev["scale"].append(h[1].to(self.device, non_blocking=True))
+                    dev["iq"].append(h[0].to(self.device, non_blocking=True).contiguous())
+                    dev["scale"].append(h[1].to(self.device, non_blocking=True).contiguous())
                 else:
-                    dev["iq"].append(h.to(self.device, non_blocking=True))
+                    dev["iq"].append(h.to(self.device, non_blocking=True).contiguous())
                     dev["scale"].append(None)
             dev["out"] = torch.empty((req.spec.n_frame, 1, req.spec.n_vox), dtype=torch.complex64,
                                      device=self.device)
@@ -525,10 +529,10 @@ class RcaServer:
 # ----------------------------------------------------------------------------
 # Synthetic request data
 # ----------------------------------------------------------------------------
-def synth_request(rid: int, spec: RequestSpec, backend, device, n_scat=200, seed=0):
+def synth_request(rid: int, spec: RequestSpec, backend, device, n_scat=200, seed=0, host=True):
     """Seeded analytic I/Q for a request (scatterers seeded by request id;
-    frames differ by a seeded per-frame phase), returned in pinned host memory
-    in the layout the backend uploads."""
+    frames differ by a seeded per-frame phase), in the layout the backend
+    uploads: in pinned host memory, or left on ``device`` with host=False."""
     from . import synth
     from .plan import quantize_fp16
     cfg = C.RcaConfig(f"req{rid}", backend.n_elem, backend.n_sample, spec.nx, spec.ny, spec.nz,
@@ -547,7 +551,8 @@ def synth_request(rid: int, spec: RequestSpec, backend, device, n_scat=200, seed
         ens = (one[None] * ph[:, None, None, None]).contiguous()
         if spec.dtype == "fp16":
             hq, sc = quantize_fp16(ens)
-            halves.append((hq.cpu().pin_memory(), sc.cpu().pin_memory()))
+            halves.append((hq.cpu().pin_memory(), sc.cpu().pin_memory()) if host else (hq, sc))
         else:
-            halves.append(ens.cpu().pin_memory())
+            halves.append(ens.cpu().pin_memory() if host else ens)
+        del ens
     return Request(rid, spec, halves)

@@ -140,14 +140,14 @@ def ensemble(cfg: C.RcaConfig, n_frame: int | None = None, device="cpu"):
     for h in cfg.halves:
         arr = torch.zeros(nf, len(h.theta), cfg.n_elem, cfg.n_sample, dtype=torch.complex64,
                           device=device)
+        # the tissue slab is static: its echoes once, then each frame adds the
+        # moving scatterers' echoes to a copy
+        base = echoes(cfg, h, tpos, tamp, device) if nt else None
         for f in range(nf):
             p = pos + vel * (f * ex.get("frame_dt", 1e-3))
-            if nt:
-                p = np.concatenate([p, tpos])
-                a = np.concatenate([amp, tamp])
-            else:
-                a = amp
-            echoes(cfg, h, p, a, device, out=arr[f])
+            if base is not None:
+                arr[f].copy_(base)
+            echoes(cfg, h, p, amp, device, out=arr[f])
         outs.append(arr)
     return outs
 
