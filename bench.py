@@ -208,6 +208,65 @@ def run_fp16(cfg, dev, n_frame=16, reps=3):
     return res
 
 
+def run_cfg2(dev):
+    """Config 2: the 200-frame Doppler ensemble on the config-1 geometry
+    (scatterers moving up to 5 mm/s, 1 ms apart, a static tissue slab 40 dB
+    brighter), both halves compounded, from FP16-I/Q (half2 + per-trace FP32
+    scale, 6.7 GB) and from complex64 I/Q (13.4 GB); updates/s of each pass
+    and the FP16 result's error against the FP32 one over all frames."""
+    import torch
+    from simu_rca_ab_amd import configs as C, synth
+    from simu_rca_ab_amd.plan import quantize_fp16
+    cfg = C.get("cfg2")
+    nf = cfg.n_frame
+    t0 = time.perf_counter()
+    halves = synth.ensemble(cfg, device=dev)
+    torch.cuda.synchronize()
+    log(f"[bench] cfg2: {nf}-frame ensemble synthesised in {time.perf_counter() - t0:.1f} s")
+    plans = build_plans(cfg, dev)
+    q = [quantize_fp16(h) for h in halves]
+    out32 = torch.empty((nf, 1, plans[0].n_pix), dtype=torch.complex64, device=dev)
+    out16 = torch.empty_like(out32)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    def run32():
+        for i, (p, x) in enumerate(zip(plans, halves)):
+            p.forward(x, out=out32, want_mask=False, accumulate=i > 0)
+
+    def run16():
+        for i, (p, (h, sc)) in enumerate(zip(plans, q)):
+            p.forward(h, sc, out=out16, want_mask=False, accumulate=i > 0)
+
+    t32 = timed(run32)
+    t16 = timed(run16)
+    upd = cfg.updates_per_frame() * nf
+    d = (out16 - out32).to(torch.complex128)
+    rel = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(out32.to(torch.complex128)))
+    per = torch.linalg.vector_norm(d[:, 0], dim=1) / torch.linalg.vector_norm(
+        out32[:, 0].to(torch.complex128), dim=1)
+    g, w = out16.abs().reshape(-1).double(), out32.abs().reshape(-1).double()
+    keep = w >= w.max() * 1e-2
+    env = float((20 * torch.log10(g[keep].clamp_min(1e-30)) - 20 * torch.log10(w[keep])).abs().max())
+    log(f"[bench] cfg2: FP32 {upd / t32 / 1e9:.1f}, FP16 {upd / t16 / 1e9:.1f} G updates/s, "
+        f"FP16 vs FP32 rel-L2 {rel:.2e}, envelope {env:.3f} dB")
+    res = {"fp32_updates_per_s": upd / t32, "fp16_updates_per_s": upd / t16,
+           "fp16_over_fp32": t32 / t16, "fp32_s": t32, "fp16_s": t16, "frames": nf,
+           "updates": upd, "fp16_vs_fp32_rel_l2": rel, "fp16_vs_fp32_rel_l2_worst_frame": float(per.max()),
+           "fp16_vs_fp32_envelope_db_40db": env,
+           "workload": "cfg2: 128+128 elements, 2 x 16 tilts, 2048 samples, 128x128x256 voxels, "
+                       "Lanczos a=4, 200 frames (1000 moving scatterers + 2000 static tissue "
+                       "scatterers 40 dB brighter), I/Q resident in HBM; the FP64-oracle error on "
+                       "whole tiles of frames 0, 99, 199 is in tests/test_gpu_fullsize.py"}
+    del halves, q, out32, out16, d
+    return res
+
+
 def run_adjoint(dev, reps=3):
     """DAS forward and adjoint (the backward of the beamformer) on config 4:
     96+96 elements, 16 tilts, 96x96x192 voxels, 8 frames, Lanczos a=3."""
@@ -408,12 +467,17 @@ def run_sim(dev, reps=2):
                         "flat elevation"}
 
 
-def run_serving(dev, capacity, n_req=32, load=0.7, seed=3000):
-    """Serving stream: Poisson arrivals of requests drawn like the serving
-    workload (grid N_xy ~ U{64..160}, N_z = 2 N_xy, tilts in {8,16,32,64},
-    FP32 or FP16 I/Q) with 1-4 frames per request, offered at ``load`` times
-    the plain kernel's measured throughput."""
+def run_serving(dev, capacity, n_req=16, load=0.7, seed=3000):
+    """Serving stream as config 3 states it: Poisson arrivals of requests on
+    the resident 128+128-element probe (2048 samples, Lanczos a=4), each with
+    grid N_xy ~ U{64..160}, N_z = 2 N_xy, tilts from {8, 16, 32, 64}, frames
+    ~ U{1..100} and FP32 or FP16 I/Q (even odds), its own seeded scatterers;
+    offered at ``load`` times ``capacity`` (updates/s).  The requests' I/Q is
+    synthesised in HBM before the stream starts (as the on-GPU simulator
+    would deliver it); latency is arrival -> beamformed volume in host
+    memory."""
     import numpy as np
+    import torch
     from simu_rca_ab_amd.serving import GpuBackend, RcaServer, RequestSpec, synth_request
     rng = np.random.Generator(np.random.PCG64(seed))
     be = GpuBackend(dev)
@@ -421,29 +485,37 @@ def run_serving(dev, capacity, n_req=32, load=0.7, seed=3000):
     for _ in range(n_req):
         nxy = int(rng.integers(64, 161))
         specs.append(RequestSpec(nxy, nxy, 2 * nxy, int(rng.choice([8, 16, 32, 64])),
-                                 int(rng.integers(1, 5)), "fp16" if rng.random() < 0.5 else "fp32"))
-    log(f"[bench] serving: synthesising {n_req} requests")
-    reqs = [synth_request(i, sp, be, dev, n_scat=100) for i, sp in enumerate(specs)]
-    mean_upd = float(np.mean([sp.updates(be.n_elem) for sp in specs]))
+                                 int(rng.integers(1, 101)), "fp16" if rng.random() < 0.5 else "fp32"))
+    log(f"[bench] serving: synthesising {n_req} requests in HBM")
+    reqs = [synth_request(i, sp, be, dev, n_scat=100, host=False) for i, sp in enumerate(specs)]
+    torch.cuda.synchronize()
+    upd = [sp.updates(be.n_elem) for sp in specs]
+    mean_upd = float(np.mean(upd))
     rate = load * capacity / mean_upd
     gaps = rng.exponential(1.0 / rate, n_req)
-    srv = RcaServer(be, budget_bytes=48 << 30).start()
-    log(f"[bench] serving: {n_req} requests at {rate:.2f} req/s (offered load {load:.0%})")
-    t0 = time.perf_counter()
-    t = t0
-    for r, g in zip(reqs, gaps):
+    srv = RcaServer(be, budget_bytes=192 << 30).start()
+    log(f"[bench] serving: {n_req} requests at {rate:.3f} req/s (offered load {load:.0%} of "
+        f"{capacity / 1e9:.0f} G updates/s)")
+    t = time.perf_counter()
+    for i, (r, g) in enumerate(zip(reqs, gaps)):
         t += g
         while time.perf_counter() < t:
             time.sleep(min(1e-3, max(0.0, t - time.perf_counter())))
         srv.submit(r)
+        if i % 4 == 3:
+            log(f"[bench] serving: {i + 1} submitted")
     for r in reqs:
-        r.wait(600)
+        r.wait(900)
     srv.close(60)
     st = srv.stats.summary()
-    st.update({"arrival_rate_per_s": rate, "offered_load": load, "n_requests": n_req,
-               "frames_per_request": "1-4", "mean_updates_per_request": mean_upd,
+    st.update({"arrival_rate_per_s": rate, "offered_load": load, "capacity_basis": capacity,
+               "n_requests": n_req, "frames_per_request": "1-100 (uniform)",
+               "grid": "N_xy ~ U{64..160}, N_z = 2 N_xy", "tilts": "{8, 16, 32, 64}",
+               "mean_updates_per_request": mean_upd, "max_updates_per_request": float(max(upd)),
                "fp16_fraction": sum(sp.dtype == "fp16" for sp in specs) / n_req,
+               "iq": "resident in HBM at arrival",
                "latency": "arrival -> beamformed volume in host memory"})
+    del reqs
     return st
 
 
@@ -575,6 +647,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-I/Q leg")
+    ap.add_argument("--no-cfg2", action="store_true", help="skip the 200-frame config-2 leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream leg")
     ap.add_argument("--no-svd", action="store_true", help="skip the SVD clutter-filter leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (cfg4) leg")
@@ -629,12 +702,20 @@ def main():
     roof = roofline(cfg, mean_launch, upd_launch, probe_t)
     roof["ceiling_two_vgpr_sources"] = probe_t0
     extra = {}
+    ens_rate = None
     if not args.no_fp16:
         log("[bench] FP16-I/Q ensemble leg (16 frames)")
         r = run_fp16(cfg, dev)
         extra["fp16_iq"] = {"value": r["fp16"], "fp32_same_ensemble": r["fp32"],
                             "unit": "voxel*tilt*element updates/s", "frames": 16,
-                            "vs_plain_fp32_single_frame": r["fp16"] / value}
+                            "vs_plain_fp32_single_frame": r["fp16"] / value,
+                            "fp16_over_fp32": r["fp16"] / r["fp32"]}
+        ens_rate = 0.5 * (r["fp16"] + r["fp32"])
+        torch.cuda.empty_cache()
+    if not args.no_cfg2:
+        log("[bench] config-2 leg (200-frame ensemble, FP16 and FP32)")
+        extra["cfg2_ensemble"] = run_cfg2(dev)
+        torch.cuda.empty_cache()
     if not args.no_adjoint:
         log("[bench] adjoint leg (config 4)")
         extra["adjoint"] = run_adjoint(dev)
@@ -656,8 +737,8 @@ def main():
         extra["dbf"] = run_dbf(dev)
         torch.cuda.empty_cache()
     if not args.no_serving:
-        log("[bench] serving leg")
-        extra["serving"] = run_serving(dev, value)
+        log("[bench] serving leg (config 3)")
+        extra["serving"] = run_serving(dev, ens_rate or value)
         torch.cuda.empty_cache()
         log("[bench] saturated FP16 serving leg")
         extra["serving_fp16_saturated"] = run_serving_saturated(dev, value)

@@ -160,15 +160,16 @@ int rca_adjoint(const rca_das_desc* desc, const void* grad_out, int grad_tx, int
  * (complex64 [n_frame][1][nPix]) and `mask` ([1][nPix], written by the batch
  * that contains frame 0).  rca_batch_encode fills a host workspace (header,
  * per-half arguments, work items); the caller copies it to a device buffer of
- * the same size; rca_batch_launch runs the batch as four launches (FP32
- * items' first halves, their second halves, then the same for FP16) of
+ * the same size; rca_batch_launch runs the batch as eight launches of
  * max_items workgroups, one work item per workgroup (max_items as given to
- * rca_batch_encode).  The launch arguments never change
+ * rca_batch_encode): for each item type - complex64 or FP16 I/Q, with or
+ * without a depth hint (rca_das_desc.depth_run >= 64) - the first halves,
+ * then the second halves.  The launch arguments never change
  * between batches, so one captured HIP graph replays every batch.  Per-voxel
  * arithmetic is the same as rca_forward's: results are bitwise identical
  * whatever else shares the batch.  Frames are processed in blocks of
- * rca_batch_frame_block(iq_dtype): an entry's frame_begin must be a multiple
- * of it, and frame_end a multiple of it or n_frame.
+ * rca_batch_frame_block(iq_dtype, depth_tiled): an entry's frame_begin must be
+ * a multiple of it, and frame_end a multiple of it or n_frame.
  */
 typedef struct {
   const rca_das_desc* half[2];
@@ -188,9 +189,10 @@ int rca_batch_encode(const rca_batch_entry* entries, int n_entries, int max_halv
                      int n_sample_max, void* host_ws, size_t ws_bytes, int* n_items);
 int rca_batch_launch(const void* dev_ws, int max_halves, int max_items, int interp,
                      int n_sample_max, hipStream_t stream);
-/* Frames per block of a batch entry of this I/Q dtype (1 for complex64); 0
- * for an unknown dtype. */
-int rca_batch_frame_block(int iq_dtype);
+/* Frames per block of a batch entry of this I/Q dtype, with (depth_tiled != 0)
+ * or without a depth hint: 8 with one, else 1 for complex64 and 4 for FP16;
+ * 0 for an unknown dtype. */
+int rca_batch_frame_block(int iq_dtype, int depth_tiled);
 
 /*
  * Launch-shape override for tuning sweeps (no reference counterpart):
@@ -241,7 +243,8 @@ int rca_plan_destroy(rca_plan* plan);
  * kernels are reported against.  n_blocks workgroups of 256 lanes, each lane
  * 8 independent chains x 8 x iters instructions of one class: mode 0
  * v_fma_f32 (VGPR multiplier), 1 v_fma_f32 (inline constants), 2 v_add_f32,
- * 3 v_mul_f32, 4 v_fma_f64, 5 v_max_f32, 6 v_add_u32.  Writes one float per
+ * 3 v_mul_f32, 4 v_fma_f64, 5 v_max_f32, 6 v_add_u32, 7 v_fma_f32 with three
+ * VGPR sources, 8 v_add_f32 and 9 v_mul_f32 with two.  Writes one float per
  * lane to out (n_blocks * 256) and the lane-operations of the launch to
  * *lane_ops.  Asynchronous on stream.
  */

@@ -40,6 +40,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "../../include/rca_das.h"
 #include "rca_interp_poly.h"
@@ -774,19 +775,22 @@ das_fwd(const KArgs a) {
 // ----------------------------------------------------------------------------
 // Batched forward for the serving loop.  The work list is kept in device
 // memory, so one captured sequence of launches serves batches of any shape:
-//   header  { n FP32 items, n FP16 items }
-//   args    KArgs per request half (pointers, sizes, flags; iq_dtype below)
+//   header  { n items of each type (4) }
+//   args    KArgs per request half (pointers, sizes, flags, tilts per chunk
+//           and row length of its item type)
 //   items   { first arg, number of halves (1|2), voxel block, frame block },
-//           the FP32 items first, then the FP16 items
-// Four launches per batch: FP32 half 0, FP32 half 1, FP16 half 0, FP16 half 1.
+//           grouped by type in type order
+// Item types: FP32 or FP16 I/Q, each with whole-trace rows (entries without
+// a depth hint: one frame per FP32 item, kBatchFB16 per FP16 item) or with
+// depth tiles (kBatchFB32Tiled / kBatchFB16Tiled frames per item, the
+// standalone frame-block kernels' layout).  Eight launches per batch: per type, half 0 then half 1.
 // Workgroup g of a launch takes item g of its type (a grid of max_items
 // workgroups; those past the item count return at once), so the second half
 // of an item accumulates onto the first in the next launch and no two
-// workgroups of a launch share an output.  One kernel per sample type and no
-// loop around the block keep each at the register count of the plain kernels
-// (a persistent kernel over both types and halves took 160 VGPRs, 3 waves per
-// SIMD, and ran FP32 items at two thirds of the plain kernel's rate).  FP32
-// items use V = kBatchV32, FB = 1; FP16 items V = 1, FB = kBatchFB16.
+// workgroups of a launch share an output.  One kernel per type and no loop
+// around the block keep each at the register count of the plain kernels (a
+// persistent kernel over both sample types and halves took 160 VGPRs, 3 waves
+// per SIMD, and ran FP32 items at two thirds of the plain kernel's rate).
 // ----------------------------------------------------------------------------
 struct BatchArgs {
   KArgs k;
@@ -794,12 +798,28 @@ struct BatchArgs {
   int pad_[3];
 };
 
-// FP16 frames per workgroup in batches (rca_batch_frame_block).
+// Frames per work item: FP16 entries without depth tiles, and every entry
+// with depth tiles (rca_batch_frame_block).
 #ifndef RCA_BATCH_FB16
 #define RCA_BATCH_FB16 4
 #endif
-constexpr int kBatchV32 = 1;
+#ifndef RCA_BATCH_FB32T
+#define RCA_BATCH_FB32T 8
+#endif
+#ifndef RCA_BATCH_FB16T
+#define RCA_BATCH_FB16T 8
+#endif
 constexpr int kBatchFB16 = RCA_BATCH_FB16;
+constexpr int kBatchFB32Tiled = RCA_BATCH_FB32T;
+constexpr int kBatchFB16Tiled = RCA_BATCH_FB16T;
+constexpr int kBatchTypes = 4;
+__host__ __device__ constexpr int batch_type(bool f32, bool tiled) { return (f32 ? 0 : 2) + (tiled ? 1 : 0); }
+__host__ __device__ constexpr bool batch_f32(int type) { return type < 2; }
+__host__ __device__ constexpr bool batch_tiled(int type) { return (type & 1) != 0; }
+__host__ __device__ constexpr int batch_fb(int type) {
+  return batch_tiled(type) ? (batch_f32(type) ? kBatchFB32Tiled : kBatchFB16Tiled)
+                           : batch_f32(type) ? 1 : kBatchFB16;
+}
 
 // A wave-uniform struct read from memory, word by word through
 // readfirstlane: the values land in scalar registers, as kernel arguments do
@@ -815,21 +835,22 @@ __device__ __forceinline__ T uniform_load(const T* p) {
   return v;
 }
 
-template <int KIND, typename S>
+template <int KIND, int TYPE>
 __global__ void __launch_bounds__(kThreads) das_fwd_batch(const int* __restrict__ header,
                                                           const BatchArgs* __restrict__ args,
                                                           const int4* __restrict__ items, int h) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  constexpr bool kF32 = sizeof(S) == 8;
-  const int n = kF32 ? header[0] : header[1];
+  using S = typename std::conditional<batch_f32(TYPE), float2, __half2>::type;
+  constexpr int FB = batch_fb(TYPE);
+  const int n = header[TYPE];
   if ((int)blockIdx.x >= n) return;
-  const int4 it = uniform_load(items + (kF32 ? 0 : header[0]) + blockIdx.x);
+  int first = 0;
+#pragma unroll
+  for (int t = 0; t < TYPE; ++t) first += header[t];
+  const int4 it = uniform_load(items + first + blockIdx.x);
   if (h >= it.y) return;
   const KArgs k = uniform_load(&args[it.x + h].k);
-  if constexpr (kF32)
-    fwd_block<KIND, kBatchV32, 1, tc_max(1, 8), true, float2>(k, it.z, it.w, smem_raw);
-  else
-    fwd_block<KIND, 1, kBatchFB16, tc_max(kBatchFB16, 4), true, __half2>(k, it.z, it.w, smem_raw);
+  fwd_block<KIND, 1, FB, tc_max(FB, sizeof(S)), true, S>(k, it.z, it.w, smem_raw);
 }
 
 // ----------------------------------------------------------------------------
@@ -1419,19 +1440,21 @@ int bfVirtualSourceGeneric_RCA(float* dataBf, float* mask, const float* data,
 // ---------------------------------------------------------------------------
 // Batched launches for the serving loop (persistent, graph-capturable).
 // ---------------------------------------------------------------------------
-// LDS of a batched launch per sample type: the standalone kernels' budget
-// (32 KB for FP32 single-frame blocks, 48 KB for FP16 frame blocks), or one
-// tilt of whole-trace rows when that is larger (FP32 V = kBatchV32, FB = 1;
-// FP16 V = 1, FB = kBatchFB16; 16-byte rows).  The FP32 budget matters: at
-// 48 KB only 3 workgroups fit a CU (3 waves per SIMD instead of 4).
-static size_t batch_lds(int n_sample_max, int iq_dtype) {
+// LDS layout of a batch item type, shared by every entry of that type so the
+// launch knows its size: with depth tiles, the standalone frame-block
+// kernels' layout (rows of kTileRow samples, 48 KB budget); with whole-trace
+// rows, the standalone budget (32 KB for FP32 one-frame items - at 48 KB only
+// 3 workgroups fit a CU - and 48 KB for FP16 frame blocks) or one tilt of
+// whole traces of n_sample_max samples when that is larger.
+static FwdLayout batch_layout(int n_sample_max, int type) {
+  const int fb = batch_fb(type);
+  const size_t sb = batch_f32(type) ? 8 : 4;
+  if (batch_tiled(type))
+    return rca_tile::fwd_layout(64, n_sample_max, 1, fb, sb, tc_max(fb, (int)sb), kFwdLds);
   const size_t ns = ((size_t)n_sample_max + 3) / 4 * 4;
-  if (iq_dtype == RCA_IQ_C64) {
-    const size_t need = (size_t)kBatchV32 * kThreads * sizeof(TxTerm) + ns * 8;
-    return need > kFwdLds1 ? need : kFwdLds1;
-  }
-  const size_t need = (size_t)kThreads * sizeof(TxTerm) + (size_t)kBatchFB16 * ns * 4;
-  return need > kFwdLds ? need : kFwdLds;
+  const size_t need = (size_t)kThreads * sizeof(TxTerm) + (size_t)fb * ns * sb;
+  const size_t dflt = fb == 1 ? kFwdLds1 : kFwdLds;
+  return rca_tile::fwd_layout(0, n_sample_max, 1, fb, sb, tc_max(fb, (int)sb), need > dflt ? need : dflt);
 }
 
 static size_t batch_args_off() { return 256; }
@@ -1454,17 +1477,23 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
   unsigned char* ws = reinterpret_cast<unsigned char*>(host_ws);
   BatchArgs* args = reinterpret_cast<BatchArgs*>(ws + batch_args_off());
   int4* items = reinterpret_cast<int4*>(ws + batch_items_off(max_halves));
-  int na = 0, ni = 0, n32 = 0;
-  // the FP32 entries' items first (pass 0), then the FP16 entries' (pass 1)
-  for (int q = 0; q < n_entries; ++q)
+  int na = 0, ni = 0;
+  int count[kBatchTypes] = {0, 0, 0, 0};
+  for (int q = 0; q < n_entries; ++q) {
     if (e[q].iq_dtype != RCA_IQ_C64 && e[q].iq_dtype != RCA_IQ_F16X2)
       return fail(RCA_ERR_INVALID, "entry %d: unknown iq dtype", q);
-  for (int pass = 0; pass < 2; ++pass)
+    if (e[q].n_half < 1 || e[q].n_half > 2)
+      return fail(RCA_ERR_INVALID, "entry %d: n_half must be 1 or 2", q);
+    if (!e[q].half[0]) return fail(RCA_ERR_INVALID, "entry %d: desc is NULL", q);
+  }
+  // the items grouped by type, in type order
+  for (int pass = 0; pass < kBatchTypes; ++pass)
   for (int q = 0; q < n_entries; ++q) {
     const rca_batch_entry& E = e[q];
-    if ((E.iq_dtype == RCA_IQ_C64) != (pass == 0)) continue;
-    if (E.n_half < 1 || E.n_half > 2) return fail(RCA_ERR_INVALID, "entry %d: n_half must be 1 or 2", q);
-    const int fb = E.iq_dtype == RCA_IQ_C64 ? 1 : kBatchFB16;
+    const bool tiled = E.half[0]->depth_run >= 64;
+    const int type = batch_type(E.iq_dtype == RCA_IQ_C64, tiled);
+    if (type != pass) continue;
+    const int fb = batch_fb(type);
     if (!E.out || E.n_frame < 1 || E.frame_begin < 0 || E.frame_end > E.n_frame ||
         E.frame_begin >= E.frame_end || E.frame_begin % fb ||
         (E.frame_end % fb && E.frame_end != E.n_frame))
@@ -1498,26 +1527,30 @@ int rca_batch_encode(const rca_batch_entry* e, int n_entries, int max_halves, in
       a.nosum = 0;
       const size_t sb = E.iq_dtype == RCA_IQ_C64 ? 8 : 4;
       a.vec_ok = (((uintptr_t)E.iq[h] & 15) == 0) && (((size_t)d->n_sample * sb) % 16 == 0);
-      const FwdLayout L = rca_tile::fwd_layout(a.depth_run, a.n_sample,
-                                               E.iq_dtype == RCA_IQ_C64 ? kBatchV32 : 1, fb, sb,
-                                               tc_max(fb, (int)sb), batch_lds(n_sample_max, E.iq_dtype));
-      a.tc = L.tc;
-      a.wcap = L.wcap;
+      if (tiled) {
+        const FwdLayout L = batch_layout(n_sample_max, type);
+        a.tc = L.tc;
+        a.wcap = L.wcap;
+      } else {   // whole traces of this entry in rows sized by the type's budget
+        const FwdLayout L = rca_tile::fwd_layout(0, a.n_sample, 1, fb, sb, tc_max(fb, (int)sb),
+                                                 batch_layout(n_sample_max, type).lds);
+        a.tc = L.tc;
+        a.wcap = L.wcap;
+      }
       args[na].k = a;
       args[na].iq_dtype = E.iq_dtype;
       ++na;
     }
-    const long long nvb = rca_tile::fwd_blocks(n_pix, depth_run, E.iq_dtype == RCA_IQ_C64 ? kBatchV32 : 1);
+    const long long nvb = rca_tile::fwd_blocks(n_pix, depth_run, 1);
     for (int fbk = E.frame_begin / fb; fbk * fb < E.frame_end; ++fbk) {
       for (long long vb = 0; vb < nvb; ++vb) {
         if (ni >= max_items) return fail(RCA_ERR_UNSUPPORTED, "batch exceeds %d items", max_items);
         items[ni++] = make_int4(first, E.n_half, (int)vb, fbk);
-        if (pass == 0) ++n32;
+        ++count[type];
       }
     }
   }
-  reinterpret_cast<int*>(ws)[0] = n32;
-  reinterpret_cast<int*>(ws)[1] = ni - n32;
+  for (int t = 0; t < kBatchTypes; ++t) reinterpret_cast<int*>(ws)[t] = count[t];
   *n_items_out = ni;
   return RCA_OK;
 }
@@ -1527,9 +1560,11 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int max_items, int inte
   g_err.clear();
   if (!dev_ws || max_halves < 1 || max_items < 1 || n_sample_max < 1)
     return fail(RCA_ERR_INVALID, "bad batch launch arguments");
-  const size_t lds32 = batch_lds(n_sample_max, RCA_IQ_C64), lds16 = batch_lds(n_sample_max, RCA_IQ_F16X2);
-  if (lds32 > (size_t)kMaxLds || lds16 > (size_t)kMaxLds)
-    return fail(RCA_ERR_UNSUPPORTED, "nSample %d exceeds LDS", n_sample_max);
+  size_t lds[kBatchTypes];
+  for (int t = 0; t < kBatchTypes; ++t) {
+    lds[t] = batch_layout(n_sample_max, t).lds;
+    if (lds[t] > (size_t)kMaxLds) return fail(RCA_ERR_UNSUPPORTED, "nSample %d exceeds LDS", n_sample_max);
+  }
   const unsigned char* ws = reinterpret_cast<const unsigned char*>(dev_ws);
   const int* hdr = reinterpret_cast<const int*>(ws);
   const BatchArgs* args = reinterpret_cast<const BatchArgs*>(ws + batch_args_off());
@@ -1539,9 +1574,13 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int max_items, int inte
 #define RCA_BATCH_CASE(K)                                                                       \
     case K:                                                                                     \
       for (int h = 0; h < 2; ++h)                                                               \
-        hipLaunchKernelGGL((das_fwd_batch<K, float2>), grid, block, lds32, stream, hdr, args, items, h); \
+        hipLaunchKernelGGL((das_fwd_batch<K, 0>), grid, block, lds[0], stream, hdr, args, items, h); \
       for (int h = 0; h < 2; ++h)                                                               \
-        hipLaunchKernelGGL((das_fwd_batch<K, __half2>), grid, block, lds16, stream, hdr, args, items, h); \
+        hipLaunchKernelGGL((das_fwd_batch<K, 1>), grid, block, lds[1], stream, hdr, args, items, h); \
+      for (int h = 0; h < 2; ++h)                                                               \
+        hipLaunchKernelGGL((das_fwd_batch<K, 2>), grid, block, lds[2], stream, hdr, args, items, h); \
+      for (int h = 0; h < 2; ++h)                                                               \
+        hipLaunchKernelGGL((das_fwd_batch<K, 3>), grid, block, lds[3], stream, hdr, args, items, h); \
       break;
     RCA_BATCH_CASE(RCA_INTERP_QUADRATIC)
     RCA_BATCH_CASE(RCA_INTERP_LANCZOS_REF)
@@ -1554,8 +1593,9 @@ int rca_batch_launch(const void* dev_ws, int max_halves, int max_items, int inte
   return err == hipSuccess ? RCA_OK : hip_fail(err, "das_fwd_batch launch");
 }
 
-int rca_batch_frame_block(int iq_dtype) {
-  return iq_dtype == RCA_IQ_C64 ? 1 : iq_dtype == RCA_IQ_F16X2 ? kBatchFB16 : 0;
+int rca_batch_frame_block(int iq_dtype, int depth_tiled) {
+  if (iq_dtype != RCA_IQ_C64 && iq_dtype != RCA_IQ_F16X2) return 0;
+  return batch_fb(batch_type(iq_dtype == RCA_IQ_C64, depth_tiled != 0));
 }
 
 }  // extern "C"

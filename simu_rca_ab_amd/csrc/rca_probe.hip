@@ -9,8 +9,11 @@
 //   0  v_fma_f32, multiplier in a VGPR      4  v_fma_f64, inline constants
 //   1  v_fma_f32, inline constants          5  v_max_f32 across chains
 //   2  v_add_f32, inline constant           6  v_add_u32 across chains
-//   3  v_mul_f32, literal constant
+//   3  v_mul_f32, literal constant          7  v_fma_f32, three VGPR sources
+//                                           8  v_add_f32, two VGPR sources
+//                                           9  v_mul_f32, two VGPR sources
 // (no packed forms: the library is built without packed-FP32 instructions).
+// Modes 7-9 add loop-invariant per-lane operands b[k], c[k] held in VGPRs.
 #include <hip/hip_runtime.h>
 
 #include "../../include/rca_das.h"
@@ -55,6 +58,27 @@ __global__ void __launch_bounds__(kProbeThreads) valu_probe(float* __restrict__ 
     }
 #pragma unroll
     for (int k = 0; k < kChains; ++k) s += (float)(a[k] & 1023u);
+  } else if constexpr (MODE >= 7) {
+    float a[kChains], b[kChains], cc[kChains];
+#pragma unroll
+    for (int k = 0; k < kChains; ++k) {
+      a[k] = (float)(threadIdx.x + k) * 1e-3f;
+      b[k] = m - (float)((threadIdx.x + k) & 7) * 1e-6f;
+      cc[k] = c + (float)((threadIdx.x * 3 + k) & 7) * 1e-7f;
+    }
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+      for (int r = 0; r < kUnroll; ++r) {
+#pragma unroll
+        for (int k = 0; k < kChains; ++k) {
+          if constexpr (MODE == 7) a[k] = fmaf(a[k], b[k], cc[k]);
+          else if constexpr (MODE == 8) a[k] = a[k] + cc[k];
+          else a[k] = a[k] * b[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kChains; ++k) s += a[k] + b[k];
   } else {
     float a[kChains];
 #pragma unroll
@@ -84,7 +108,7 @@ extern "C" {
 
 int rca_probe_valu(int mode, float* out, int n_blocks, int iters, hipStream_t stream,
                    long long* lane_ops) {
-  if (!out || n_blocks < 1 || iters < 1 || mode < 0 || mode > 6) {
+  if (!out || n_blocks < 1 || iters < 1 || mode < 0 || mode > 9) {
     rca_internal::set_error("rca_probe_valu: bad arguments");
     return RCA_ERR_INVALID;
   }
@@ -96,7 +120,10 @@ int rca_probe_valu(int mode, float* out, int n_blocks, int iters, hipStream_t st
     case 3: hipLaunchKernelGGL(valu_probe<3>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
     case 4: hipLaunchKernelGGL(valu_probe<4>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
     case 5: hipLaunchKernelGGL(valu_probe<5>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
-    default: hipLaunchKernelGGL(valu_probe<6>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
+    case 6: hipLaunchKernelGGL(valu_probe<6>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
+    case 7: hipLaunchKernelGGL(valu_probe<7>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
+    case 8: hipLaunchKernelGGL(valu_probe<8>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
+    default: hipLaunchKernelGGL(valu_probe<9>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {

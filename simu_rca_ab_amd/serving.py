@@ -20,8 +20,10 @@ FP32 scale).  The server
   launch after its first), whose work list is copied
   into a fixed device buffer right before the replay — two work-list slots
   keep one batch queued while the previous one runs;
-* downloads each finished request's volume and records its latency
-  (arrival -> result in host memory).
+* hands each finished request to a download thread, which copies its volume
+  to pinned host memory and records its latency (arrival -> result in host
+  memory); pinning a multi-GB result buffer takes long enough that doing it
+  on the scheduler thread would leave the GPU without queued batches.
 
 Per-voxel arithmetic does not depend on what else is in a batch, so a
 request's result is bitwise identical alone and in any mix.
@@ -33,6 +35,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import queue
 import threading
 import time
 from collections import deque
@@ -144,16 +147,20 @@ class GpuBackend:
         lib.rca_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_void_p]
         lib.rca_batch_launch.restype = ctypes.c_int
-        lib.rca_batch_frame_block.argtypes = [ctypes.c_int]
-        # FP16 frames run in blocks of this many per workgroup: chunks start on a block
-        self.fb16 = int(lib.rca_batch_frame_block(_lib.IQ_F16X2))
+        lib.rca_batch_frame_block.argtypes = [ctypes.c_int, ctypes.c_int]
+        # frames run in blocks per workgroup (by dtype and depth tiling):
+        # chunks start on a block
+        self.fb = {(dt, tiled): int(lib.rca_batch_frame_block(code, int(tiled)))
+                   for dt, code in (("fp32", _lib.IQ_C64), ("fp16", _lib.IQ_F16X2))
+                   for tiled in (False, True)}
         self.device = torch.device(device)
         self.n_elem, self.n_sample, self.interp = n_elem, n_sample, interp
         self.max_halves, self.max_items = max_halves, max_items
         self.fs, self.fd, self.c, self.t0, self.fnumber = fs, fd, c, t0, fnumber
         self.dxyz, self.z0, self.max_tilt_deg = dxyz, z0, max_tilt_deg
         self.compute = torch.cuda.Stream(self.device)
-        self.copy = torch.cuda.Stream(self.device)
+        self.copy = torch.cuda.Stream(self.device)       # uploads
+        self.d2h = torch.cuda.Stream(self.device)        # downloads (never delay an upload)
         self._grids = {}
         self._tilts = {}
         self._descs = {}
@@ -296,12 +303,12 @@ class GpuBackend:
         return ev, n_items.value
 
     def download(self, req: Request, after):
-        with torch.cuda.stream(self.copy):
-            self.copy.wait_event(after)
-            host = torch.empty((req.spec.n_frame, req.spec.n_vox), dtype=torch.complex64).pin_memory()
+        host = torch.empty((req.spec.n_frame, req.spec.n_vox), dtype=torch.complex64, pin_memory=True)
+        with torch.cuda.stream(self.d2h):
+            self.d2h.wait_event(after)
             host.copy_(req.dev["out"][:, 0], non_blocking=True)
             ev = torch.cuda.Event()
-            ev.record(self.copy)
+            ev.record(self.d2h)
         return host, ev
 
     def release(self, req: Request):
@@ -309,16 +316,17 @@ class GpuBackend:
 
     def max_frames_per_chunk(self, spec: RequestSpec, quantum: int) -> int:
         """Frames per work chunk: about ``quantum`` updates, within the work-list
-        capacity; a multiple of the FP16 frame block for FP16 requests (the
-        batched kernel runs FP16 frames in blocks, so chunks must start on a
-        block)."""
+        capacity, a multiple of the request's frame block (the batched kernels
+        run frames in blocks - 8 with depth tiles - so chunks start on a
+        block; the last chunk ends at the last frame)."""
         per_frame = spec.updates(self.n_elem) // spec.n_frame
-        items_per_frame = math.ceil(spec.n_vox / 256)   # a work item tiles 16 depths x 16 columns
-        f = max(1, min(quantum // max(per_frame, 1), self.max_items // max(items_per_frame, 1)))
-        if spec.dtype == "fp16":
-            b = self.fb16
-            f = max(b, f - f % b)
-        return f
+        tiled = spec.nz >= 64                   # the descriptor's depth hint is Nz
+        b = self.fb[(spec.dtype, tiled)]
+        # work items per frame block: 16-depth x 16-column tiles, or 256 voxels
+        items = (math.ceil(spec.nx * spec.ny / 16) * math.ceil(spec.nz / 16) if tiled
+                 else math.ceil(spec.n_vox / 256))
+        f = min(quantum // max(per_frame, 1), (self.max_items // max(items, 1)) * b)
+        return max(b, f - f % b)
 
 
 class RcaServer:
@@ -336,7 +344,8 @@ class RcaServer:
         self.stats = ServerStats()
         self.slots = [None, None]        # (event, work) per work-list slot
         self.next_slot = 0
-        self.downloads = []              # (req, host, event)
+        self._dl = queue.Queue()         # (req, batch event) to download, None ends
+        self._dl_thread = None
         self._stop = False
         self._closed = False
         self._thread = None
@@ -356,6 +365,8 @@ class RcaServer:
         return req
 
     def start(self):
+        self._dl_thread = threading.Thread(target=self._download_loop, daemon=True)
+        self._dl_thread.start()
         self._thread = threading.Thread(target=self._loop, daemon=True)
         self._thread.start()
         return self
@@ -395,7 +406,7 @@ class RcaServer:
         t0 = time.perf_counter()
         while True:
             with self.lock:
-                busy = bool(self.queue) or bool(self.active) or bool(self.downloads)
+                busy = bool(self.queue) or bool(self.active)
             if not busy:
                 return
             if timeout is not None and time.perf_counter() - t0 > timeout:
@@ -429,7 +440,9 @@ class RcaServer:
 
     def _build_batch(self):
         work, budget = [], self.quantum
-        runnable = [r for r in self.active if r.frames_issued < r.spec.n_frame
+        with self.lock:
+            active = list(self.active)
+        runnable = [r for r in active if r.frames_issued < r.spec.n_frame
                     and r.dev.get("uploaded") is not None and r.dev["uploaded"].query()]
         # least attained service first: a request that has had less GPU work so
         # far (e.g. one that just arrived) is served before long-running ones
@@ -461,32 +474,47 @@ class RcaServer:
         ev.synchronize()
         self.slots[k] = None
         now = time.perf_counter()
+        with self.lock:
+            for r, f0, f1 in work:
+                r.frames_done += f1 - f0
+                self.stats.updates += r.spec.updates(self.be.n_elem) // r.spec.n_frame * (f1 - f0)
+            self.stats.t_last = max(self.stats.t_last or now, now)
         for r, f0, f1 in work:
-            r.frames_done += f1 - f0
-            self.stats.updates += r.spec.updates(self.be.n_elem) // r.spec.n_frame * (f1 - f0)
             if r.frames_done == r.spec.n_frame:
-                host, dev_ev = self.be.download(r, ev)
-                self.downloads.append((r, host, dev_ev))
-        self.stats.t_last = now
+                self._dl.put((r, ev))
 
-    def _finish_downloads(self, block=False):
-        keep = []
-        for r, host, ev in self.downloads:
-            if block:
-                ev.synchronize()
-            if ev.query():
+    def _download_loop(self):
+        """Download thread: volume to host memory, latency, release."""
+        while True:
+            item = self._dl.get()
+            if item is None:
+                return
+            r, ev = item
+            try:
+                host, dev_ev = self.be.download(r, ev)
+                dev_ev.synchronize()
                 r.result = host
                 r.done = time.perf_counter()
-                self.stats.latencies.append(r.latency())
-                self.stats.t_last = r.done
+            except Exception as exc:   # the request fails; the scheduler goes on
+                r.error = f"download failed: {exc!r}"
+                r.done = time.perf_counter()
+            try:
                 self.be.release(r)
-                with self.lock:
-                    self.in_use -= r.dev_bytes
+            except Exception:
+                pass
+            with self.lock:
+                if r.error is None:
+                    self.stats.latencies.append(r.latency())
+                    self.stats.t_last = max(self.stats.t_last or r.done, r.done)
+                self.in_use -= r.dev_bytes
+                if r in self.active:
                     self.active.remove(r)
-                r.event.set()
-            else:
-                keep.append((r, host, ev))
-        self.downloads = keep
+            r.event.set()
+
+    def _stop_downloads(self):
+        if self._dl_thread is not None:
+            self._dl.put(None)
+            self._dl_thread.join()
 
     def _loop(self):
         try:
@@ -510,19 +538,18 @@ class RcaServer:
                 for j in (0, 1):
                     if self.slots[j] is not None and self.slots[j][0].query():
                         self._retire(j)
-                self._finish_downloads()
                 if not launched:
                     time.sleep(self.idle_sleep)
             for j in (0, 1):
                 if self.slots[j] is not None:
                     self._retire(j)
-            self._finish_downloads(block=True)
+            self._stop_downloads()
             self._fail_pending("server closed before the request finished")
         except Exception as exc:  # surface scheduler failures to every waiter
             with self.lock:
                 self.error = repr(exc)
                 self._closed = True
-            self.downloads = []
+            self._stop_downloads()
             self._fail_pending(f"server failed: {exc!r}")
 
 

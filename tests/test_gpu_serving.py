@@ -92,7 +92,8 @@ def test_serving_matches_plain_path_and_oracle(backend, requests, dev):
         ref = out[:, 0].cpu().numpy()
         r_rel = H.rel_l2(got.numpy(), ref)
         say(f"request {r.rid} ({spec.dtype}): rel vs plain path {r_rel:.2e}")
-        assert r_rel <= 1e-6
+        # frame blocks set the tilts per chunk: the tilt-sum order may differ
+        assert r_rel <= 4e-6
         if spec.dtype == "fp32":
             # every voxel of the small grids; whole workgroup tiles of the tiled ones
             sel = (np.arange(spec.n_vox) if spec.nz < 64 else

@@ -183,24 +183,29 @@ def test_close_fails_queued_and_unfinished_requests():
     assert not srv.queue and not srv.active
 
 
-def test_gpu_backend_chunks_start_on_fp16_frame_blocks():
-    """GpuBackend.max_frames_per_chunk (host logic only): FP16 chunks are a
-    multiple of the batched kernel's FP16 frame block, at least one block,
-    FP32 chunks any size >= 1; both within the quantum and the work-list
-    capacity when a frame fits them."""
+def test_gpu_backend_chunks_start_on_frame_blocks():
+    """GpuBackend.max_frames_per_chunk (host logic only): chunks are a
+    multiple of the batched kernels' frame block of the request (by dtype and
+    depth tiling), at least one block; within the quantum when a block fits
+    it, and within the work-list capacity (work items of 16x16-voxel tiles
+    with depth tiles, 256 voxels without)."""
+    import math
     from types import SimpleNamespace
     from simu_rca_ab_amd.serving import GpuBackend
-    for fb16 in (2, 4):
-        be = SimpleNamespace(n_elem=128, max_items=1 << 17, fb16=fb16)
-        for nxy, nt, nf in ((64, 8, 100), (100, 16, 40), (160, 64, 7), (128, 32, 8)):
+    for fbt in (4, 8):
+        fb = {("fp32", False): 1, ("fp16", False): 4, ("fp32", True): fbt, ("fp16", True): fbt}
+        be = SimpleNamespace(n_elem=128, max_items=1 << 17, fb=fb)
+        for nxy, nz, nt, nf in ((64, 128, 8, 100), (100, 200, 16, 40), (160, 320, 64, 7),
+                                (128, 256, 32, 8), (24, 40, 8, 9), (117, 234, 32, 100)):
             for dt in ("fp32", "fp16"):
-                sp = RequestSpec(nxy, nxy, 2 * nxy, nt, nf, dt)
+                sp = RequestSpec(nxy, nxy, nz, nt, nf, dt)
                 q = int(1.5e10)
                 f = GpuBackend.max_frames_per_chunk(be, sp, q)
                 per = sp.updates(be.n_elem) // sp.n_frame
-                assert f >= 1
-                if dt == "fp16":
-                    assert f % fb16 == 0 and f >= fb16
-                if per <= q and dt == "fp32":
+                b = fb[(dt, nz >= 64)]
+                assert f >= b and f % b == 0
+                if per * b <= q:
                     assert f * per <= q
-                assert f * -(-sp.n_vox // 256) <= be.max_items or f <= fb16
+                items = (math.ceil(nxy * nxy / 16) * math.ceil(nz / 16) if nz >= 64
+                         else math.ceil(sp.n_vox / 256))
+                assert (f // b) * items <= be.max_items or f == b
