@@ -244,7 +244,8 @@ int rca_plan_destroy(rca_plan* plan);
  * 8 independent chains x 8 x iters instructions of one class: mode 0
  * v_fma_f32 (VGPR multiplier), 1 v_fma_f32 (inline constants), 2 v_add_f32,
  * 3 v_mul_f32, 4 v_fma_f64, 5 v_max_f32, 6 v_add_u32, 7 v_fma_f32 with three
- * VGPR sources, 8 v_add_f32 and 9 v_mul_f32 with two.  Writes one float per
+ * VGPR sources, 8 v_add_f32 and 9 v_mul_f32 with two, 10 v_fma_f32 with an
+ * SGPR and two VGPRs, 11 v_fma_f32 with two VGPRs and a literal.  Writes one float per
  * lane to out (n_blocks * 256) and the lane-operations of the launch to
  * *lane_ops.  Asynchronous on stream.
  */

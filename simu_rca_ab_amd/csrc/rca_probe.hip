@@ -12,6 +12,8 @@
 //   3  v_mul_f32, literal constant          7  v_fma_f32, three VGPR sources
 //                                           8  v_add_f32, two VGPR sources
 //                                           9  v_mul_f32, two VGPR sources
+//                                          10  v_fma_f32, SGPR + two VGPRs
+//                                          11  v_fma_f32, two VGPRs + literal
 // (no packed forms: the library is built without packed-FP32 instructions).
 // Modes 7-9 add loop-invariant per-lane operands b[k], c[k] held in VGPRs.
 #include <hip/hip_runtime.h>
@@ -73,7 +75,9 @@ __global__ void __launch_bounds__(kProbeThreads) valu_probe(float* __restrict__ 
         for (int k = 0; k < kChains; ++k) {
           if constexpr (MODE == 7) a[k] = fmaf(a[k], b[k], cc[k]);
           else if constexpr (MODE == 8) a[k] = a[k] + cc[k];
-          else a[k] = a[k] * b[k];
+          else if constexpr (MODE == 9) a[k] = a[k] * b[k];
+          else if constexpr (MODE == 10) a[k] = fmaf(a[k], m, cc[k]);
+          else a[k] = fmaf(a[k], b[k], 0.123f);
         }
       }
     }
@@ -108,7 +112,7 @@ extern "C" {
 
 int rca_probe_valu(int mode, float* out, int n_blocks, int iters, hipStream_t stream,
                    long long* lane_ops) {
-  if (!out || n_blocks < 1 || iters < 1 || mode < 0 || mode > 9) {
+  if (!out || n_blocks < 1 || iters < 1 || mode < 0 || mode > 11) {
     rca_internal::set_error("rca_probe_valu: bad arguments");
     return RCA_ERR_INVALID;
   }
@@ -123,7 +127,9 @@ int rca_probe_valu(int mode, float* out, int n_blocks, int iters, hipStream_t st
     case 6: hipLaunchKernelGGL(valu_probe<6>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
     case 7: hipLaunchKernelGGL(valu_probe<7>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
     case 8: hipLaunchKernelGGL(valu_probe<8>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
-    default: hipLaunchKernelGGL(valu_probe<9>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
+    case 9: hipLaunchKernelGGL(valu_probe<9>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
+    case 10: hipLaunchKernelGGL(valu_probe<10>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
+    default: hipLaunchKernelGGL(valu_probe<11>, g, b, 0, stream, out, iters, 0.999f, 1e-3f); break;
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
