@@ -189,6 +189,20 @@ int rca_batch_encode(const rca_batch_entry* entries, int n_entries, int max_halv
                      int n_sample_max, void* host_ws, size_t ws_bytes, int* n_items);
 int rca_batch_launch(const void* dev_ws, int max_halves, int max_items, int interp,
                      int n_sample_max, hipStream_t stream);
+/*
+ * rca_batch_launch without a caller-allocated workspace (no reference
+ * counterpart; for callers that cannot allocate device memory: code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Takes the work list as rca_batch_encode wrote it in host memory
+ * (host_ws, ws_bytes >= rca_batch_workspace_bytes(max_halves, max_items)),
+ * copies it on `stream` into a device array the library holds per GPU
+ * (4 MiB: rca_batch_workspace_bytes(max_halves, max_items) must not exceed
+ * it, else RCA_ERR_UNSUPPORTED) and runs the batch from there.  Allocates
+ * nothing.  Calls that overlap on one GPU share that array and race; calls on
+ * one stream are ordered.
+ */
+int rca_batch_launch_noalloc(const void* host_ws, size_t ws_bytes, int max_halves, int max_items,
+                             int interp, int n_sample_max, hipStream_t stream);
 /* Frames per block of a batch entry of this I/Q dtype, with (depth_tiled != 0)
  * or without a depth hint: 8 with one, else 1 for complex64 and 4 for FP16;
  * 0 for an unknown dtype. */

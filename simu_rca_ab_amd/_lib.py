@@ -23,7 +23,8 @@ EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last
            "rca_version", "rca_set_tuning", "rca_batch_workspace_bytes", "rca_batch_encode",
            "rca_batch_launch", "rca_batch_frame_block", "rca_svd_workspace_bytes", "rca_svd_filter", "rca_sim_txfield",
            "rca_sim_rxfield", "rca_sim_receive", "rca_sim_spectra_to_raw", "rca_prepare_geometry",
-           "rca_plan_create", "rca_plan_desc", "rca_plan_destroy", "rca_probe_valu")
+           "rca_plan_create", "rca_plan_desc", "rca_plan_destroy", "rca_probe_valu",
+           "rca_batch_launch_noalloc")
 
 
 class RcaDasDesc(ctypes.Structure):
