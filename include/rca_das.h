@@ -266,6 +266,17 @@ int rca_plan_destroy(rca_plan* plan);
 int rca_probe_valu(int mode, float* out, int n_blocks, int iters, hipStream_t stream,
                    long long* lane_ops);
 
+/*
+ * FP16-I/Q storage (no reference counterpart: the reference keeps I/Q in
+ * FP32, bfVirtualSourceGeneric_RCA.m:476-484).  iq: complex64 traces
+ * [n_trace][n_sample]; out: half2 [n_trace][n_sample]; scale: float
+ * [n_trace].  Per trace, scale = max(|re|, |im|) over its samples (1 for an
+ * all-zero trace) and each sample is stored as (re, im) / scale rounded to
+ * the nearest half.  Asynchronous on stream.
+ */
+int rca_quantize_fp16(const void* iq, long long n_trace, int n_sample, void* out, float* scale,
+                      hipStream_t stream);
+
 /* Last error message of the calling thread ("" when none). */
 const char* rca_last_error(void);
 

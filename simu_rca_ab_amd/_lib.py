@@ -24,7 +24,7 @@ EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last
            "rca_batch_launch", "rca_batch_frame_block", "rca_svd_workspace_bytes", "rca_svd_filter", "rca_sim_txfield",
            "rca_sim_rxfield", "rca_sim_receive", "rca_sim_spectra_to_raw", "rca_prepare_geometry",
            "rca_plan_create", "rca_plan_desc", "rca_plan_destroy", "rca_probe_valu",
-           "rca_batch_launch_noalloc")
+           "rca_batch_launch_noalloc", "rca_quantize_fp16")
 
 
 class RcaDasDesc(ctypes.Structure):
@@ -98,7 +98,8 @@ def load():
     lib.rca_plan_destroy.argtypes = [vp]
     lib.rca_sim_spectra_to_raw.argtypes = [vp, i, i, i, i, vp, vp]
     lib.rca_probe_valu.argtypes = [i, vp, i, i, vp, ctypes.POINTER(ll)]
-    for name in ("rca_probe_valu", "rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA",
+    lib.rca_quantize_fp16.argtypes = [vp, ll, i, vp, vp, vp]
+    for name in ("rca_quantize_fp16", "rca_probe_valu", "rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA",
                  "rca_svd_filter", "rca_sim_txfield", "rca_sim_rxfield", "rca_sim_receive",
                  "rca_sim_spectra_to_raw", "rca_prepare_geometry", "rca_plan_create",
                  "rca_plan_destroy"):

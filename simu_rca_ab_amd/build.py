@@ -12,7 +12,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 SOURCES = [PKG / "csrc" / n for n in ("rca_das.hip", "rca_plan.hip", "rca_svd.hip", "rca_sim.hip",
-                                      "rca_probe.hip", "rca_noalloc.hip")]
+                                      "rca_probe.hip", "rca_noalloc.hip", "rca_quant.hip")]
 OUT = PKG / "lib" / "librca_das.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",

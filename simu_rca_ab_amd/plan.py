@@ -97,21 +97,22 @@ def _dev_f32(a, device):
 
 
 def quantize_fp16(iq: torch.Tensor):
-    """FP16-I/Q storage: half2 samples scaled by a per-trace FP32 factor.
-
-    scale[f, t, c] = max over the trace of max(|re|, |im|) (1 for an all-zero
-    trace); samples are stored as (re, im) / scale in float16.  Returns
-    (float16 [F, T, C, Ns, 2], float32 [F, T, C]).
-    """
-    r = torch.view_as_real(iq)
-    scale = r.abs().amax(dim=(-1, -2)).to(torch.float32)
-    scale = torch.where(scale > 0, scale, torch.ones_like(scale))
-    h = (r / scale[..., None, None]).to(torch.float16).contiguous()
-    return h, scale.contiguous()
-
-
-def dequantize_fp16(h: torch.Tensor, scale: torch.Tensor) -> torch.Tensor:
-    return torch.view_as_complex((h.to(torch.float32) * scale[..., None, None]).contiguous())
+    """FP16-I/Q storage on the GPU (``rca_quantize_fp16``): half2 samples
+    scaled by a per-trace FP32 factor, scale[f, t, c] = max over the trace of
+    max(|re|, |im|) (1 for an all-zero trace), samples (re, im) / scale
+    rounded to float16.  iq: complex64 [F, T, C, Ns] on a GPU.  Returns
+    (float16 [F, T, C, Ns, 2], float32 [F, T, C])."""
+    if iq.dtype != torch.complex64 or not iq.is_cuda or iq.dim() < 1:
+        raise ValueError("quantize_fp16 takes complex64 I/Q on a GPU")
+    iq = iq.contiguous()
+    ns = iq.shape[-1]
+    h = torch.empty(tuple(iq.shape) + (2,), dtype=torch.float16, device=iq.device)
+    scale = torch.empty(tuple(iq.shape[:-1]), dtype=torch.float32, device=iq.device)
+    lib = _lib.load()
+    _lib.check(lib.rca_quantize_fp16(ctypes.c_void_p(iq.data_ptr()), iq.numel() // max(ns, 1), ns,
+                                     ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(scale.data_ptr()),
+                                     ctypes.c_void_p(torch.cuda.current_stream(iq.device).cuda_stream)))
+    return h, scale
 
 
 class RcaPlan:

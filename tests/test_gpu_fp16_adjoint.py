@@ -17,7 +17,8 @@ import torch
 
 from oracle import rca_oracle as O
 from simu_rca_ab_amd import _lib, configs as C
-from simu_rca_ab_amd.plan import dequantize_fp16, quantize_fp16
+from simu_rca_ab_amd.plan import quantize_fp16
+from tests.torch_ref import dequantize_fp16, quantize_fp16 as quantize_fp16_ref
 from tests import helpers as H
 from tests.test_gpu_forward import check, oracle_fwd, plan_for, say
 
@@ -45,6 +46,26 @@ def test_fp16_iq(dev, half_index):
     deq = dequantize_fp16(h, s).cpu()
     refq, _ = oracle_fwd(CFG, half_index, deq, "lanczos4")
     check(out16.cpu().numpy(), refq.sum(axis=1, keepdims=True), "fp16 vs FP64 oracle (dequantised)")
+
+
+def test_quantize_kernel_matches_torch_statement(dev):
+    """rca_quantize_fp16 against the torch statement of the storage format,
+    bit for bit: traces of every magnitude, an all-zero trace, a trace whose
+    largest component is imaginary, a length that is not a multiple of the
+    workgroup."""
+    say("case FP16-I/Q quantisation kernel vs torch statement")
+    g = torch.Generator().manual_seed(5)
+    x = torch.complex(torch.randn((3, 4, 5, 1000), generator=g), torch.randn((3, 4, 5, 1000), generator=g))
+    x = x * torch.logspace(-8, 8, 60).reshape(3, 4, 5, 1)
+    x[1, 2, 3] = 0
+    x[2, 0, 1, 17] = 1j * 1e3
+    x = x.to(torch.complex64).to(dev)
+    h, s = quantize_fp16(x)
+    hr, sr = quantize_fp16_ref(x)
+    torch.cuda.synchronize()
+    assert h.shape == hr.shape and s.shape == sr.shape
+    assert torch.equal(s, sr) and torch.equal(h, hr)
+    assert s[1, 2, 3].item() == 1.0 and not h[1, 2, 3].any()
 
 
 ADJ = C.scaled(C.get("cfg0"), nx=8, ny=6, nz=24)

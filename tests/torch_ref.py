@@ -56,3 +56,20 @@ def das_torch(iq, idx, coef):
         g = flat[:, lin.reshape(-1)].reshape(nf, *lin.shape)       # [F, P, C, K]
         out.append((g * coef[t][None]).sum(dim=(-1, -2)))
     return torch.stack(out, dim=1)
+
+
+def quantize_fp16(iq):
+    """Torch statement of the FP16-I/Q storage: per-trace scale =
+    max(|re|, |im|) (1 for a zero trace), samples (re, im) / scale rounded to
+    float16.  Returns (float16 [..., Ns, 2], float32 [...])."""
+    import torch
+    r = torch.view_as_real(iq)
+    scale = r.abs().amax(dim=(-1, -2)).to(torch.float32)
+    scale = torch.where(scale > 0, scale, torch.ones_like(scale))
+    return (r / scale[..., None, None]).to(torch.float16).contiguous(), scale.contiguous()
+
+
+def dequantize_fp16(h, scale):
+    """The complex64 values FP16-I/Q storage represents."""
+    import torch
+    return torch.view_as_complex((h.to(torch.float32) * scale[..., None, None]).contiguous())
