@@ -210,7 +210,7 @@ int rca_batch_frame_block(int iq_dtype, int depth_tiled);
 
 /*
  * Launch-shape override for tuning sweeps (no reference counterpart):
- * voxels per lane v in {1,2,4}, frames per workgroup fb in {1,2,4,8} (forward
+ * voxels per lane v in {1,2,4} (forward and adjoint), frames per workgroup fb in {1,2,4,8} (forward
  * and adjoint), split: reserved (>= 0, ignored), LDS budget of a workgroup in
  * KiB (16..64, which sets how many tilts share one pass over the receive
  * lines); 0 restores the automatic choice.  Process-wide.

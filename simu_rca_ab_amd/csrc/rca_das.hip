@@ -881,6 +881,15 @@ __global__ void __launch_bounds__(kThreads) das_fwd_batch(const int* __restrict_
   fwd_block<KIND, 1, FB, tc_max(FB, sizeof(S)), true, S>(k, it.z, it.w, smem_raw);
 }
 
+// Workgroup barrier that orders LDS only: waits for the wave's own LDS
+// accesses, not for its global stores and atomics (the adjoint's flush
+// atomics have no reader inside the workgroup, and waiting for them at every
+// barrier stalls the workgroup on L2 atomic latency).  The memory clobber
+// keeps the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // ----------------------------------------------------------------------------
 // Adjoint kernel (the transpose of fwd_block): 256 lanes, one voxel each,
 // frames [by*FB, by*FB + FB).  Same depth tiles (one column per wave: the 64
@@ -907,7 +916,7 @@ __global__ void __launch_bounds__(kThreads) das_fwd_batch(const int* __restrict_
 // that lands on another lane's tap k sees that lane's write.  When two active
 // lanes share a first tap (a claim slot per first tap detects it) the tilt's
 // taps go through LDS float atomics instead, which are slower but safe.
-template <int KIND, int FB, bool PT>
+template <int KIND, int FB, bool PT, int V>
 __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, float* rw,
                                             int* claim, unsigned act, int nt, float er, float ei,
                                             int ri, float rf, int cb, bool last, int step, int k,
@@ -919,7 +928,7 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
   const int ns = a.n_sample;
   const int lane = threadIdx.x & 63;
 #pragma unroll 1
-  for (int j = 0; j < nt; ++j, tv += kThreads, rw += FB * 2 * wcap) {
+  for (int j = 0; j < nt; ++j, tv += kThreads * V, rw += FB * 2 * wcap) {
     if (!((act >> j) & 1u)) continue;
     const TxTerm x = *tv;
     int m;
@@ -984,7 +993,7 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
   }
 }
 
-template <int KIND, int FB, int TC, bool PX>
+template <int KIND, int FB, int TC, bool PX, int V>
 __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned by,
                                           unsigned char* smem_raw) {
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
@@ -1006,7 +1015,7 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
   const int wcap = a.wcap;
   const int tcn = a.tc;   // tilts per chunk (<= TC)
   // per wave: rows [j][fi][wcap] of complex samples; then per wave a claim slot per sample
-  float* rows = reinterpret_cast<float*>(smem_raw + (size_t)tcn * kThreads * sizeof(TxTerm));
+  float* rows = reinterpret_cast<float*>(smem_raw + (size_t)tcn * kThreads * V * sizeof(TxTerm));
   const size_t wrows = (size_t)tcn * FB * 2 * wcap;   // floats per wave
   float* myrows = rows + wave * wrows;
   int* claim = reinterpret_cast<int*>(rows + 4 * wrows) + wave * wcap;
@@ -1017,28 +1026,36 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
   const bool gt1 = a.gout_tx == 1;
   int line_no = 0;   // lines visited, alternates the s_pxr slot
 
-  double pl[1], pt[1], pz[1], apF[1], iapF[1];
-  bool valid[1];
-  const TileBox box = tile_voxels<1, 1, 1>(a, bx, tid, rx, pl, pt, pz, apF, iapF, valid, s_box);
-  bool ok;
-  const long long p = voxel_of<1, 1>(a, bx, 1, 0, tid, ok);
-  // the voxel's output gradient per frame (compounded output)
-  float2 g[FB];
+  double pl[V], pt[V], pz[V], apF[V], iapF[V];
+  bool valid[V];
+  const TileBox box = tile_voxels<V, 1, 1>(a, bx, tid, rx, pl, pt, pz, apF, iapF, valid, s_box);
+  long long p[V];
+  float2 g[V][FB];   // the voxels' output gradients per frame (compounded output)
 #pragma unroll
-  for (int fi = 0; fi < FB; ++fi)
-    g[fi] = (valid[0] && gt1 && fi < nf) ? a.gout[(long long)(f0 + fi) * a.n_pix + p]
-                                         : make_float2(0.0f, 0.0f);
+  for (int v = 0; v < V; ++v) {
+    bool ok;
+    p[v] = voxel_of<1, 1>(a, bx, V, v, tid, ok);
+#pragma unroll
+    for (int fi = 0; fi < FB; ++fi)
+      g[v][fi] = (valid[v] && gt1 && fi < nf) ? a.gout[(long long)(f0 + fi) * a.n_pix + p[v]]
+                                              : make_float2(0.0f, 0.0f);
+  }
   const int row = tid / G, li = tid % G;   // clearing / flushing assignment
   const int rj = row / FB, rfi = row % FB;
+  const int c_first = (int)(((bx + (unsigned)by * 7919u) * 2654435761u) >> 8) % a.n_ch;
 
   for (int t0 = 0; t0 < a.n_tx; t0 += tcn) {
     const int nt = min(tcn, a.n_tx - t0);
-    unsigned act[1];
-    chunk_tx<1>(a, t0, nt, rx, pl, pt, pz, valid, txc, tid, act);
+    unsigned act[V];
+    chunk_tx<V>(a, t0, nt, rx, pl, pt, pz, valid, txc, tid, act);
     float bu_lo, bu_hi;
     chunk_tx_bounds(a, box, t0, nt, lane, rx, bu_lo, bu_hi);
 
-    for (int c = 0; c < a.n_ch; ++c) {
+    for (int ci = 0; ci < a.n_ch; ++ci) {
+      // workgroups start at different lines, so the tiles resident at one
+      // time add to different traces instead of contending for the same
+      // samples in L2
+      const int c = (ci + c_first) % a.n_ch;
       const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
       const double e_lat = rx ? exd : eyd;
       double uabc = 0.0, pabc = 0.0;
@@ -1047,11 +1064,12 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
         uabc = a.fs * ab;
         pabc = a.fd * ab;
       }
-      double lawv[1] = {0.0};   // PIXEL_CHANNEL law of the voxel on this line (s)
+      double lawv[V];   // PIXEL_CHANNEL law of the voxels on this line (s)
       float px_lo = 0.0f, px_hi = 0.0f;
       if constexpr (ab_px) {
-        if (valid[0]) lawv[0] = pixel_law(a, c, p);
-        pixel_law_range<1>(a, lawv, act, valid, s_pxr, line_no & 1, tid, px_lo, px_hi);
+#pragma unroll
+        for (int v = 0; v < V; ++v) lawv[v] = valid[v] ? pixel_law(a, c, p[v]) : 0.0;
+        pixel_law_range<V>(a, lawv, act, valid, s_pxr, line_no & 1, tid, px_lo, px_hi);
         ++line_no;
       }
       int s0, whi;
@@ -1066,7 +1084,7 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
         const int len = min(cb + wcap, whi) - cb;
         // clear every wave's rows [j][fi] (the previous flush has read them),
         // and publish the TX_CHANNEL law's per-tilt delays and phasors
-        __syncthreads();
+        lds_barrier();
         if (rj < nt && rfi < nf) {
           for (int w = 0; w < 4; ++w) {
             float2* r = reinterpret_cast<float2*>(rows + w * wrows + (size_t)row * 2 * wcap);
@@ -1081,15 +1099,19 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
           split_pos(a.fs * ab, s_uab_i[tid], s_uab_f[tid]);
           s_eab[tid] = make_float2(cs, sn);
         }
-        __syncthreads();
+        lds_barrier();
 
-        const double A = fabs(pl[0] - e_lat);   // .cu:284-291
-        if (act[0] && A < apF[0]) {             // .cu:294
-          const float w = hann((float)(A * iapF[0]));   // .cu:299
-          const double dz = pz[0] - ezd;
+        // the lane's voxels one after another (neighbouring columns, same
+        // depth): each adds to the same rows, flushed once for all of them
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const double A = fabs(pl[v] - e_lat);   // .cu:284-291
+          if (!(act[v] && A < apF[v])) continue;  // .cu:294
+          const float w = hann((float)(A * iapF[v]));   // .cu:299
+          const double dz = pz[v] - ezd;
           const double Rr = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));   // .cu:308-309
-          const double uab = ab_px ? a.fs * lawv[0] : uabc;
-          const double pab = ab_px ? a.fd * lawv[0] : pabc;
+          const double uab = ab_px ? a.fs * lawv[v] : uabc;
+          const double pab = ab_px ? a.fd * lawv[v] : pabc;
           int ri;
           float rf;
           split_pos(fma(a.kfs, Rr, uab), ri, rf);                          // .cu:310-311
@@ -1097,16 +1119,17 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
           float sn, cs;
           sincos2pi((float)(ph - rint(ph)), sn, cs);
           const float er = w * cs, ei = w * sn;
+          const TxTerm* tv = txc + v * kThreads + tid;
           if (gt1)
-            adj_scatter<KIND, FB, false>(a, txc + tid, myrows, claim, act[0], nt, er, ei, ri, rf, cb,
-                                         last, step, k, wcap, nf, g, f0, t0, p, ab_tc, s_uab_i, s_uab_f,
-                                         s_eab);
+            adj_scatter<KIND, FB, false, V>(a, tv, myrows, claim, act[v], nt, er, ei, ri, rf, cb, last,
+                                            step, k, wcap, nf, g[v], f0, t0, p[v], ab_tc, s_uab_i,
+                                            s_uab_f, s_eab);
           else
-            adj_scatter<KIND, FB, true>(a, txc + tid, myrows, claim, act[0], nt, er, ei, ri, rf, cb,
-                                        last, step, k, wcap, nf, g, f0, t0, p, ab_tc, s_uab_i, s_uab_f,
-                                        s_eab);
+            adj_scatter<KIND, FB, true, V>(a, tv, myrows, claim, act[v], nt, er, ei, ri, rf, cb, last,
+                                           step, k, wcap, nf, g[v], f0, t0, p[v], ab_tc, s_uab_i,
+                                           s_uab_f, s_eab);
         }
-        __syncthreads();
+        lds_barrier();
         // add the four waves' rows to the gradient traces (other tiles add to
         // the same samples)
         if (rj < nt && rfi < nf) {
@@ -1128,10 +1151,10 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
   __syncthreads();
 }
 
-template <int KIND, int FB, bool PX>
+template <int KIND, int FB, bool PX, int V>
 __global__ void __launch_bounds__(kThreads) das_adj(const KArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  adj_block<KIND, FB, tc_max(FB, 8), PX>(a, blockIdx.x, blockIdx.y, smem_raw);
+  adj_block<KIND, FB, tc_max(FB, 8), PX, V>(a, blockIdx.x, blockIdx.y, smem_raw);
 }
 
 __global__ void fill_zero(float* __restrict__ p, long long n) {
@@ -1290,34 +1313,36 @@ int pick_v(long long, int, int, int, bool) {
   return 1;
 }
 
-template <int KIND, int FB, bool PX = false>
+template <int KIND, int FB, bool PX = false, int V = 1>
 int launch_adj_t(KArgs a, hipStream_t st) {
   const AdjLayout L = rca_tile::adj_layout(a.depth_run, a.n_sample, FB, tc_max(FB, 8),
-                                           lds_budget(rca_tile::kAdjLds));
+                                           lds_budget(rca_tile::kAdjLds), V);
   if (L.lds > (size_t)kMaxLds || L.wcap < Interp<KIND>::LO + Interp<KIND>::HI + 1)
     return fail(RCA_ERR_UNSUPPORTED, "FB=%d adjoint rows do not fit in LDS", FB);
   a.tc = L.tc;
   a.wcap = L.wcap;
-  const dim3 grid((unsigned)rca_tile::tile_blocks<1, 1>(a.n_pix, a.depth_run, 1),
+  const dim3 grid((unsigned)rca_tile::tile_blocks<1, 1>(a.n_pix, a.depth_run, V),
                   (unsigned)((a.n_frame + FB - 1) / FB));
-  hipLaunchKernelGGL((das_adj<KIND, FB, PX>), grid, dim3(kThreads), L.lds, st, a);
+  hipLaunchKernelGGL((das_adj<KIND, FB, PX, V>), grid, dim3(kThreads), L.lds, st, a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "das_adj launch");
 }
 
 template <int KIND>
-int launch_adj_k(const KArgs& a, int FB, hipStream_t st) {
-  if (a.ab_mode == RCA_ABLAW_PIXEL_CHANNEL) {   // instantiated for FB = 1 only
-    if (FB != 1) return fail(RCA_ERR_UNSUPPORTED, "no pixel-law adjoint kernel for FB=%d", FB);
+int launch_adj_k(const KArgs& a, int FB, int V, hipStream_t st) {
+  if (a.ab_mode == RCA_ABLAW_PIXEL_CHANNEL) {   // instantiated for FB = 1, V = 1 only
+    if (FB != 1 || V != 1)
+      return fail(RCA_ERR_UNSUPPORTED, "no pixel-law adjoint kernel for FB=%d V=%d", FB, V);
     return launch_adj_t<KIND, 1, true>(a, st);
   }
-  switch (FB) {
-    case 1: return launch_adj_t<KIND, 1>(a, st);
-    case 2: return launch_adj_t<KIND, 2>(a, st);
-    case 4: return launch_adj_t<KIND, 4>(a, st);
-    case 8: return launch_adj_t<KIND, 8>(a, st);
-  }
-  return fail(RCA_ERR_UNSUPPORTED, "no adjoint kernel for FB=%d", FB);
+#define RCA_ADJ_CASE(FF, VV) \
+  if (FB == FF && V == VV) return launch_adj_t<KIND, FF, false, VV>(a, st);
+  RCA_ADJ_CASE(1, 1) RCA_ADJ_CASE(1, 2) RCA_ADJ_CASE(1, 4)
+  RCA_ADJ_CASE(2, 1) RCA_ADJ_CASE(2, 2) RCA_ADJ_CASE(2, 4)
+  RCA_ADJ_CASE(4, 1) RCA_ADJ_CASE(4, 2) RCA_ADJ_CASE(4, 4)
+  RCA_ADJ_CASE(8, 1) RCA_ADJ_CASE(8, 2)
+#undef RCA_ADJ_CASE
+  return fail(RCA_ERR_UNSUPPORTED, "no adjoint kernel for FB=%d V=%d", FB, V);
 }
 
 }  // namespace
@@ -1415,11 +1440,15 @@ int rca_adjoint(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_
   a.gin = reinterpret_cast<float2*>(grad_iq);
   a.gout_tx = grad_tx;
   a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
-  // one frame per workgroup measured fastest (more tilts per chunk fit LDS)
-  const int tfb = g_tune_fb.load();
-  int fb = tfb ? tfb : 1;
+  // frames per workgroup and voxels per lane: FB = 4 with V = 4 measured
+  // fastest on a 96x96x192 volume (8 frames); fewer frames take V = 2, which
+  // keeps the kernel at 3 waves per SIMD
+  const int tfb = g_tune_fb.load(), tv = g_tune_v.load();
+  int fb = tfb ? tfb : 4;
   while (fb > 1 && fb > n_frame) fb >>= 1;
-  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) fb = 1;   // the instantiated pixel-law shape
+  int v = tv ? tv : (fb == 4 ? 4 : 2);
+  if (fb == 8 && v > 2) v = 2;
+  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) fb = v = 1;   // the instantiated pixel-law shape
   // every workgroup adds its windows to the traces: start from zero
   if (!a.accumulate) {
     const long long n = 2LL * n_frame * d->n_tx * d->n_channel * d->n_sample;
@@ -1430,10 +1459,10 @@ int rca_adjoint(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_
   }
   if (d->n_pix == 0) return RCA_OK;
   switch (d->interp) {
-    case RCA_INTERP_QUADRATIC: return launch_adj_k<RCA_INTERP_QUADRATIC>(a, fb, stream);
-    case RCA_INTERP_LANCZOS_REF: return launch_adj_k<RCA_INTERP_LANCZOS_REF>(a, fb, stream);
-    case RCA_INTERP_LANCZOS3: return launch_adj_k<RCA_INTERP_LANCZOS3>(a, fb, stream);
-    case RCA_INTERP_LANCZOS4: return launch_adj_k<RCA_INTERP_LANCZOS4>(a, fb, stream);
+    case RCA_INTERP_QUADRATIC: return launch_adj_k<RCA_INTERP_QUADRATIC>(a, fb, v, stream);
+    case RCA_INTERP_LANCZOS_REF: return launch_adj_k<RCA_INTERP_LANCZOS_REF>(a, fb, v, stream);
+    case RCA_INTERP_LANCZOS3: return launch_adj_k<RCA_INTERP_LANCZOS3>(a, fb, v, stream);
+    case RCA_INTERP_LANCZOS4: return launch_adj_k<RCA_INTERP_LANCZOS4>(a, fb, v, stream);
   }
   return fail(RCA_ERR_INVALID, "unknown interpolation %d", d->interp);
 }

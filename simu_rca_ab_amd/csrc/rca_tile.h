@@ -209,21 +209,24 @@ inline FwdLayout fwd_layout(int depth_run, int n_sample, int V, int fb, size_t s
   return L;
 }
 
-// Adjoint: per tilt of a chunk 256 transmit terms and, for each of the 4
+// Adjoint: per tilt of a chunk 256 V transmit terms and, for each of the 4
 // waves, FB rows of complex FP32 accumulators (interleaved re, im); then a
-// claim slot per row sample for each wave.  Depth tiles (64 depths x 4
-// columns) use rows of kAdjTileRow samples, other tiles whole traces; rows are
-// capped so one tilt fits 64 KB (longer windows take several passes).
+// claim slot per row sample for each wave.  Depth tiles (64 depths x 4V
+// columns, V voxels per lane in neighbouring columns) use rows of kAdjTileRow
+// samples, other tiles whole traces; rows are capped so one tilt fits 64 KB
+// (longer windows take several passes).
 constexpr int kAdjTileRow = 192;
 constexpr size_t kAdjLds = 48 * 1024;
 
 struct AdjLayout { int tc, wcap; size_t lds; };
-inline AdjLayout adj_layout(int depth_run, int n_sample, int fb, int tc_kernel, size_t budget) {
+inline AdjLayout adj_layout(int depth_run, int n_sample, int fb, int tc_kernel, size_t budget,
+                            int V = 1) {
   AdjLayout L;
+  const size_t txc = (size_t)kThreads * V * sizeof(TxTerm);   // transmit terms of one tilt
   const int w0 = depth_run && n_sample > kAdjTileRow ? kAdjTileRow : n_sample;
-  const int wmax = (int)((kMaxLds - kThreads * sizeof(TxTerm)) / (4 * 8 * (size_t)fb + 4 * 4));
+  const int wmax = (int)((kMaxLds - txc) / (4 * 8 * (size_t)fb + 4 * 4));
   L.wcap = w0 < wmax ? w0 : wmax;
-  const size_t per_tilt = kThreads * sizeof(TxTerm) + 4 * 8 * (size_t)fb * L.wcap;
+  const size_t per_tilt = txc + 4 * 8 * (size_t)fb * L.wcap;
   const size_t claims = 4 * 4 * (size_t)L.wcap;
   const long long fit = budget > claims ? (long long)((budget - claims) / per_tilt) : 0;
   L.tc = (int)(fit < 1 ? 1 : fit > tc_kernel ? tc_kernel : fit);

@@ -35,9 +35,12 @@ def _ok(r, what, passes=None):
 
 # the launch layouts: forward FP32 one frame (the drop-in), forward FP32 and
 # FP16 frame blocks of 8 (ensembles), the serving batch kernels (FP32 one
-# frame at 32 KiB, FP16 blocks of 4 at 48 KiB), the adjoint
+# frame at 32 KiB, FP16 blocks of 4 at 48 KiB), the adjoint (4 frames x 4
+# voxels per lane, the default; 2 x 2 for fewer frames; one voxel per lane)
 LAYOUTS = {"fwd_f32": dict(), "fwd_f32_fb8": dict(fb=8), "fwd_f16_fb8": dict(sbytes=4, fb=8),
-           "batch_f16_fb4": dict(sbytes=4, fb=4, lds_kib=48), "adjoint": dict(kernel="adjoint")}
+           "batch_f16_fb4": dict(sbytes=4, fb=4, lds_kib=48),
+           "adjoint": dict(kernel="adjoint", fb=4, v=4), "adjoint_fb2_v2": dict(kernel="adjoint", fb=2, v=2),
+           "adjoint_v1": dict(kernel="adjoint")}
 
 
 @pytest.mark.parametrize("interp", ["quadratic", "lanczos", "lanczos3", "lanczos4"])
@@ -66,8 +69,8 @@ def _sample_blocks(nblk, ndc, n_rand, seed):
 def test_cfg1_other_layouts(layout):
     cfg = C.get("cfg1")
     kw = LAYOUTS[layout]
-    if layout == "adjoint":      # 64 depths x 4 columns
-        nblk, ndc = (128 * 128 // 4) * 4, 4
+    if layout == "adjoint":      # 64 depths x 16 columns
+        nblk, ndc = (128 * 128 // 16) * 4, 4
     else:                        # 16 depths x 16 columns
         nblk, ndc = (128 * 128 // 16) * 16, 16
     for half in (0, 1):
@@ -75,7 +78,7 @@ def test_cfg1_other_layouts(layout):
             f"cfg1 half {half} {layout} (sampled workgroups)")
 
 
-@pytest.mark.parametrize("layout", ["fwd_f32", "fwd_f32_fb8", "adjoint"])
+@pytest.mark.parametrize("layout", ["fwd_f32", "fwd_f32_fb8", "adjoint", "adjoint_fb2_v2"])
 def test_cfg4_full(layout):
     """96x96x192 voxels, 16 tilts x 96 lines, 1024 samples, Lanczos a=3."""
     _ok(W.check(C.get("cfg4"), 0, **LAYOUTS[layout]), f"cfg4 {layout}")
@@ -138,13 +141,14 @@ def test_lds_budgets(kib):
     """Tilts per chunk set by the LDS budget (rca_set_tuning)."""
     cfg = C.scaled(C.get("cfg1"), nx=40, ny=40)
     _ok(W.check(cfg, 1, lds_kib=kib), f"cfg1 40x40x256 forward LDS {kib} KiB")
-    _ok(W.check(cfg, 1, lds_kib=kib, kernel="adjoint"), f"cfg1 40x40x256 adjoint LDS {kib} KiB")
+    _ok(W.check(cfg, 1, lds_kib=kib, **LAYOUTS["adjoint"]), f"cfg1 40x40x256 adjoint LDS {kib} KiB")
+    _ok(W.check(cfg, 1, lds_kib=kib, **LAYOUTS["adjoint_v1"]), f"cfg1 40x40x256 adjoint V=1 LDS {kib} KiB")
 
 
 def test_no_depth_hint_whole_trace_rows():
     """Without the hint a workgroup owns 256 consecutive voxels and rows hold
     whole traces from the window start."""
-    for layout in ("fwd_f32", "adjoint"):
+    for layout in ("fwd_f32", "adjoint", "adjoint_v1"):
         _ok(W.check(C.get("cfg0"), 0, depth_run=0, **LAYOUTS[layout]), f"cfg0 no hint {layout}")
 
 
@@ -152,5 +156,5 @@ def test_narrowed_window_is_caught():
     """The check fails when the windows lose one sample of margin per side."""
     r = W.check(C.get("cfg0"), 0, shrink=1)
     assert r["violations"] > 0
-    r = W.check(C.get("cfg0"), 0, shrink=2, kernel="adjoint")
+    r = W.check(C.get("cfg0"), 0, shrink=2, **LAYOUTS["adjoint"])
     assert r["violations"] > 0

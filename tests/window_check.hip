@@ -73,7 +73,7 @@ int rca_window_check(const WinCase* wc, WinStats* st) {
   st->first_bx = -1;
   const int depth_run = K.depth_run >= 64 ? K.depth_run : 0;
   const bool fwd = K.kernel == 0;
-  const int V = fwd ? K.v : 1;
+  const int V = K.v;
   int tc, wcap, E;
   if (fwd) {
     const size_t budget = K.lds_budget ? (size_t)K.lds_budget : (K.fb == 1 ? kFwdLds1 : kFwdLds);
@@ -84,7 +84,7 @@ int rca_window_check(const WinCase* wc, WinStats* st) {
     E = 16 / K.sbytes;
   } else {
     const size_t budget = K.lds_budget ? (size_t)K.lds_budget : kAdjLds;
-    const AdjLayout L = adj_layout(depth_run, K.n_sample, K.fb, tc_max(K.fb, 8), budget);
+    const AdjLayout L = adj_layout(depth_run, K.n_sample, K.fb, tc_max(K.fb, 8), budget, V);
     tc = L.tc;
     wcap = L.wcap;
     E = 1;
@@ -95,7 +95,7 @@ int rca_window_check(const WinCase* wc, WinStats* st) {
   st->wcap = wcap;
   st->step = step;
   st->e = E;
-  const long long nblk = fwd ? fwd_blocks(K.n_pix, depth_run, V) : tile_blocks<1, 1>(K.n_pix, depth_run, 1);
+  const long long nblk = fwd ? fwd_blocks(K.n_pix, depth_run, V) : tile_blocks<1, 1>(K.n_pix, depth_run, V);
   const long long nchk = K.blocks ? K.n_blocks : nblk;
   const bool rx = K.receive_x != 0;
   // the kernels' FP64 constants (make_args)
@@ -131,7 +131,7 @@ int rca_window_check(const WinCase* wc, WinStats* st) {
           const int i = v * kThreads + tid;
           bool ok;
           const long long p = fwd ? voxel_of<kTileCols, 4>(depth_run, K.n_pix, bx, V, v, tid, ok)
-                                  : voxel_of<1, 1>(depth_run, K.n_pix, bx, 1, 0, tid, ok);
+                                  : voxel_of<1, 1>(depth_run, K.n_pix, bx, V, v, tid, ok);
           pidx[i] = p;
           valid[i] = ok;
           if (!ok) continue;
