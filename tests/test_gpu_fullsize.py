@@ -3,7 +3,7 @@
 Grids given as [Nz, Nx, Ny] volumes (or a depth hint of Nz) make every
 launch depth-tiled: a forward workgroup owns 16 depths x 16 neighbouring
 columns and stages per (tilt chunk, line) only its sample window; the adjoint
-owns 64 depths x 4 columns.  These tests run that launch at the workloads'
+owns 64 depths x 4V columns (V voxels per lane, 4 by default).  These tests run that launch at the workloads'
 own sizes:
 
 * config 0 (64+64 elements, 11 tilts, 64x64x128, a=3): the whole volume
