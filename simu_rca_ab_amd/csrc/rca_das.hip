@@ -115,6 +115,7 @@ struct KArgs {
   int gout_tx;
   int depth_run, wcap;   // voxel tiling hint; LDS samples per row
   int tc;                // tilts per chunk of the forward kernel
+  int adj_lo;            // adjoint: tiles whose columns share the transmit terms go to das_adj_lo
 };
 
 // ----------------------------------------------------------------------------
@@ -993,6 +994,29 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
   }
 }
 
+// Adjoint tiles (64 depths x 4V columns) whose columns lie along the receive
+// axis: at every depth lane, every valid voxel of the tile has column 0's
+// transmit-lateral coordinate and depth, so the transmit terms
+// (bfVirtualSourceGeneric_RCA.cu:256-269 drops the receive-axis offset) are
+// those of one column.  Column 0 of a tile is valid whenever any of its
+// columns is.  Uniform over the workgroup; both adjoint kernels evaluate it
+// the same way, so each tile is taken by exactly one of them.
+template <int V>
+__device__ __forceinline__ bool tile_shares_tx(const double* pt, const double* pz, const bool* valid,
+                                               int tid, float (*s_ptz)[64]) {
+  const int lane = tid & 63;
+  if (tid < 64) {
+    s_ptz[0][lane] = valid[0] ? (float)pt[0] : NAN;
+    s_ptz[1][lane] = (float)pz[0];
+  }
+  __syncthreads();
+  bool same = true;
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+    if (valid[v]) same = same && (float)pt[v] == s_ptz[0][lane] && (float)pz[v] == s_ptz[1][lane];
+  return __syncthreads_and(same) != 0;
+}
+
 template <int KIND, int FB, int TC, bool PX, int V>
 __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned by,
                                           unsigned char* smem_raw) {
@@ -1029,6 +1053,10 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
   double pl[V], pt[V], pz[V], apF[V], iapF[V];
   bool valid[V];
   const TileBox box = tile_voxels<V, 1, 1>(a, bx, tid, rx, pl, pt, pz, apF, iapF, valid, s_box);
+  if constexpr (V == 4 && !PX) {
+    __shared__ float s_ptz[2][64];
+    if (a.adj_lo && tile_shares_tx<V>(pt, pz, valid, tid, s_ptz)) return;   // das_adj_lo's tile
+  }
   long long p[V];
   float2 g[V][FB];   // the voxels' output gradients per frame (compounded output)
 #pragma unroll
@@ -1149,6 +1177,210 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
     }
   }
   __syncthreads();
+}
+
+// Line-outer adjoint of the tiles whose columns share the transmit terms
+// (tile_shares_tx; a tile of 64 depths x 16 columns along the receive axis,
+// 4 voxels per lane).  The transmit terms of a tilt chunk (up to
+// kAdjLoTilts tilts) are one column's: 64 per tilt, cached in LDS.  The
+// receive terms of a voxel (aperture, Hann weight, FP64 receive distance,
+// sample offset, receive phasor) are computed once per (line, chunk) instead
+// of once per (line, tilt), and the rows hold one tilt at a time: per (line,
+// tilt) the workgroup clears its rows, scatters (adj_scatter), and flushes
+// them to the gradient with global float atomics as adj_block does.
+// LDS: [tc][64] TxTerm, then per wave FB rows of wcap complex samples, then
+// per wave a claim slot per row sample.
+constexpr int kAdjLoTilts = 32;
+
+template <int KIND, int FB>
+__device__ __forceinline__ void adj_block_lo(const KArgs& a, unsigned bx, unsigned by,
+                                             unsigned char* smem_raw) {
+  constexpr int V = 4;
+  constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
+  constexpr int G = kThreads / FB;    // threads per row when clearing and flushing
+  __shared__ float s_box[4][6];
+  __shared__ float s_ptz[2][64];
+  __shared__ int s_uab_i[1];          // TX_CHANNEL law of the current (tilt, line)
+  __shared__ float s_uab_f[1];
+  __shared__ float2 s_eab[1];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int f0 = by * FB;
+  const int nf = min(FB, a.n_frame - f0);
+  const int ns = a.n_sample;
+  const int wcap = a.wcap;
+  const int tcn = a.tc;   // tilts per chunk (<= kAdjLoTilts)
+  TxTerm* txc = reinterpret_cast<TxTerm*>(smem_raw);
+  float* rows = reinterpret_cast<float*>(smem_raw + (size_t)tcn * 64 * sizeof(TxTerm));
+  const size_t wrows = (size_t)FB * 2 * wcap;   // floats per wave
+  float* myrows = rows + wave * wrows;
+  int* claim = reinterpret_cast<int*>(rows + 4 * wrows) + wave * wcap;
+  const int step = pass_step(wcap, NT, 1);
+  const bool rx = true;   // the receive_x half (transmit terms follow y and z)
+  const bool ab_tc = a.ab_mode == RCA_ABLAW_TX_CHANNEL;
+  const bool gt1 = a.gout_tx == 1;
+
+  double pl[V], pt[V], pz[V], apF[V], iapF[V];
+  bool valid[V];
+  const TileBox box = tile_voxels<V, 1, 1>(a, bx, tid, rx, pl, pt, pz, apF, iapF, valid, s_box);
+  if (!tile_shares_tx<V>(pt, pz, valid, tid, s_ptz)) return;   // das_adj's tile
+  // column 0's transmit-lateral coordinate and depth at this lane's depth
+  const float pt0 = s_ptz[0][lane], pz0 = s_ptz[1][lane];
+  const bool valid0 = !(pt0 != pt0);
+  long long p[V];
+  float2 g[V][FB];   // the voxels' output gradients per frame (compounded output)
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    bool ok;
+    p[v] = voxel_of<1, 1>(a, bx, V, v, tid, ok);
+#pragma unroll
+    for (int fi = 0; fi < FB; ++fi)
+      g[v][fi] = (valid[v] && gt1 && fi < nf) ? a.gout[(long long)(f0 + fi) * a.n_pix + p[v]]
+                                              : make_float2(0.0f, 0.0f);
+  }
+  const int row = tid / G, li = tid % G;   // clearing / flushing assignment (row = frame)
+  const int c_first = (int)(((bx + (unsigned)by * 7919u) * 2654435761u) >> 8) % a.n_ch;
+
+  for (int t0 = 0; t0 < a.n_tx; t0 += tcn) {
+    const int nt = min(tcn, a.n_tx - t0);
+    lds_barrier();   // the previous chunk's transmit terms are no longer read
+    // transmit terms of column 0 (.cu:256-269), wave w taking tilts w, w + 4, ...
+    for (int j = wave; j < nt; j += 4) {
+      const int t = t0 + j;
+      TxTerm x;
+      x.bi = 0;
+      x.bf = 0.0f;
+      x.et = make_float2(1.0f, 0.0f);
+      if (valid0) {
+        const double dl = (double)pt0 - (double)a.vy[t], dz = (double)pz0 - (double)a.vz[t];
+        const double tt = sqrt_nr(fma(dl, dl, dz * dz)) - (double)a.d0[t];
+        const double Bp = fma(a.kfd, fma(-2.0, (double)pz0, tt), a.pt0);
+        float sn, cs;
+        sincos2pi((float)(Bp - rint(Bp)), sn, cs);
+        split_pos(fma(a.kfs, tt, a.ut0), x.bi, x.bf);
+        x.et = make_float2(cs, sn);
+      }
+      txc[j * 64 + lane] = x;
+    }
+    // transmit cone bits of the lane's voxels for the chunk's tilts
+    unsigned act[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) act[v] = 0u;
+    for (int j = 0; j < nt; ++j) {
+      const int t = t0 + j;
+      const float vxf = a.vx[t], vyf = a.vy[t], vzf = a.vz[t];
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        if (valid[v] && (!a.mask_tx || tx_mask(a, vxf, vyf, vzf, (float)pl[v], (float)pt[v], (float)pz[v])))
+          act[v] |= 1u << j;
+    }
+    // transmit sample-offset bounds over the tile, lane j holding tilt t0 + j
+    float bu_lo = INFINITY, bu_hi = -INFINITY;
+    if (lane < nt)
+      rca_tile::tx_bounds(a.kfs, a.ut0, box.txl_lo, box.txl_hi, box.zlo, box.zhi, a.vy[t0 + lane],
+                          a.vz[t0 + lane], a.d0[t0 + lane], bu_lo, bu_hi);
+    lds_barrier();   // transmit terms published
+
+    for (int ci = 0; ci < a.n_ch; ++ci) {
+      const int c = (ci + c_first) % a.n_ch;
+      const double exd = a.ex[c], ezd = a.ez[c];
+      double uabc = 0.0, pabc = 0.0;
+      if (a.ab_mode == RCA_ABLAW_CHANNEL) {
+        const double ab = a.ab[c];
+        uabc = a.fs * ab;
+        pabc = a.fd * ab;
+      }
+      float r_lo, r_hi;
+      if (!rca_tile::rx_bounds(a.kfs, box, exd, ezd, uabc, r_lo, r_hi)) continue;
+      // receive terms of the lane's voxels on this line (.cu:284-311, 371-373)
+      bool inap[V];
+      int ri[V];
+      float rf[V], er[V], ei[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const double A = fabs(pl[v] - exd);
+        inap[v] = valid[v] && A < apF[v];
+        ri[v] = 0;
+        rf[v] = er[v] = ei[v] = 0.0f;
+        if (inap[v]) {
+          const float w = hann((float)(A * iapF[v]));
+          const double dz = pz[v] - ezd;
+          const double Rr = sqrt_nr(fmax(fma(A, A, dz * dz), 1e-300));
+          split_pos(fma(a.kfs, Rr, uabc), ri[v], rf[v]);
+          const double ph = fma(a.kfd, Rr, pabc);
+          float sn, cs;
+          sincos2pi((float)(ph - rint(ph)), sn, cs);
+          er[v] = w * cs;
+          ei[v] = w * sn;
+        }
+      }
+      for (int j = 0; j < nt; ++j) {
+        const int t = t0 + j;
+        int s0, whi;
+        if (!line_window<LO, HI, 1>(a, box, exd, ezd, uabc, __shfl(bu_lo, j, 64), __shfl(bu_hi, j, 64),
+                                    ab_tc, t, 1, c, lane, s0, whi))
+          continue;
+        const int npass = pass_count(s0, whi, wcap, step);
+        for (int k = 0; k < npass; ++k) {
+          const int cb = s0 + k * step;
+          const bool last = cb + wcap >= whi;
+          const int len = min(cb + wcap, whi) - cb;
+          lds_barrier();   // the previous flush has read the rows
+          if (row < nf) {
+            for (int w = 0; w < 4; ++w) {
+              float2* r = reinterpret_cast<float2*>(rows + w * wrows + (size_t)row * 2 * wcap);
+              for (int i = li; i < len; i += G) r[i] = make_float2(0.0f, 0.0f);
+            }
+          }
+          if (ab_tc && k == 0 && tid == 0) {
+            const double ab = a.ab[(long long)t * a.n_ch + c];
+            const double pab = a.fd * ab;
+            float sn, cs;
+            sincos2pi((float)(pab - rint(pab)), sn, cs);
+            split_pos(a.fs * ab, s_uab_i[0], s_uab_f[0]);
+            s_eab[0] = make_float2(cs, sn);
+          }
+          lds_barrier();
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            if (!(inap[v] && ((act[v] >> j) & 1u))) continue;
+            const TxTerm* tv = txc + j * 64 + lane;
+            if (gt1)
+              adj_scatter<KIND, FB, false, V>(a, tv, myrows, claim, 1u, 1, er[v], ei[v], ri[v], rf[v], cb,
+                                              last, step, k, wcap, nf, g[v], f0, t, p[v], ab_tc, s_uab_i,
+                                              s_uab_f, s_eab);
+            else
+              adj_scatter<KIND, FB, true, V>(a, tv, myrows, claim, 1u, 1, er[v], ei[v], ri[v], rf[v], cb,
+                                             last, step, k, wcap, nf, g[v], f0, t, p[v], ab_tc, s_uab_i,
+                                             s_uab_f, s_eab);
+          }
+          lds_barrier();
+          // the four waves' rows added to the gradient traces of tilt t
+          if (row < nf) {
+            const float2* r = reinterpret_cast<const float2*>(rows + (size_t)row * 2 * wcap);
+            const size_t wr2 = wrows / 2;
+            float* dst = reinterpret_cast<float*>(
+                a.gin + (((long long)(f0 + row) * a.n_tx + t) * a.n_ch + c) * ns + cb);
+            for (int i = li; i < len; i += G) {
+              const float2 v0 = r[i], v1 = r[wr2 + i], v2 = r[2 * wr2 + i], v3 = r[3 * wr2 + i];
+              const float vr = v0.x + v1.x + v2.x + v3.x;
+              const float vi = v0.y + v1.y + v2.y + v3.y;
+              if (vr != 0.0f) unsafeAtomicAdd(dst + 2 * i, vr);
+              if (vi != 0.0f) unsafeAtomicAdd(dst + 2 * i + 1, vi);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int KIND, int FB>
+__global__ void __launch_bounds__(kThreads) das_adj_lo(const KArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  adj_block_lo<KIND, FB>(a, blockIdx.x, blockIdx.y, smem_raw);
 }
 
 template <int KIND, int FB, bool PX, int V>
@@ -1323,6 +1555,26 @@ int launch_adj_t(KArgs a, hipStream_t st) {
   a.wcap = L.wcap;
   const dim3 grid((unsigned)rca_tile::tile_blocks<1, 1>(a.n_pix, a.depth_run, V),
                   (unsigned)((a.n_frame + FB - 1) / FB));
+  if constexpr (V == 4 && !PX) {
+    // receive-x half with depth tiles: tiles along the receive axis share
+    // the transmit terms and go to the line-outer kernel, the rest (checked
+    // per tile, so any grid is handled) to das_adj
+    if (a.receive_x && a.depth_run >= 64) {
+      KArgs b = a;
+      b.tc = a.n_tx < kAdjLoTilts ? a.n_tx : kAdjLoTilts;
+      b.wcap = a.n_sample > rca_tile::kAdjTileRow ? rca_tile::kAdjTileRow : a.n_sample;
+      const auto lo_lds = [&](int tc) {
+        return (size_t)tc * 64 * sizeof(TxTerm) + 4 * 8 * (size_t)FB * b.wcap + 4 * 4 * (size_t)b.wcap;
+      };
+      while (b.tc > 1 && lo_lds(b.tc) > (size_t)kMaxLds) --b.tc;
+      if (lo_lds(b.tc) <= (size_t)kMaxLds && b.wcap >= Interp<KIND>::LO + Interp<KIND>::HI + 1) {
+        a.adj_lo = b.adj_lo = 1;
+        hipLaunchKernelGGL((das_adj_lo<KIND, FB>), grid, dim3(kThreads), lo_lds(b.tc), st, b);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "das_adj_lo launch");
+      }
+    }
+  }
   hipLaunchKernelGGL((das_adj<KIND, FB, PX, V>), grid, dim3(kThreads), L.lds, st, a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? RCA_OK : hip_fail(e, "das_adj launch");

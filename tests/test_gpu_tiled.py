@@ -1,7 +1,7 @@
 """GPU parity of the depth-tiled forward launch (grid hint ``depth_run``).
 
 With the hint, a forward workgroup owns 16 depths x 16V neighbouring columns
-(the adjoint 64 depths x 4 columns) and stages only the sample window that its
+(the adjoint 64 depths x 4V columns) and stages only the sample window that its
 voxels' taps can reach on each (tilt chunk, line).  Its
 results must match the untiled launch and the FP64 oracle (1e-5, 0.1 dB).
 Tiled and untiled launches add the same per-voxel terms, but the tilts per
@@ -168,6 +168,73 @@ def test_tiled_adjoint(dev, interp, half_index):
     ref = O.kernel_adjoint(g.cpu().numpy(), H.grid_f32(G), prep,
                            H.oracle_params(TILE, TILE.halves[half_index]), nf, kind, a)
     r = H.rel_l2(outs[(0, 0)], ref)
+    say(f"tiled adjoint vs FP64 oracle: rel-L2 {r:.3e}")
+    assert r <= 1e-5
+
+
+# 72 columns in rows of 24: of the adjoint's 16-column tiles (4 voxels per
+# lane) those along one grid row (columns 0-15, 32-47, 48-63 and the partial
+# 64-71) share the transmit terms and go to the line-outer kernel on the
+# receive-x half (half 0); the tile spanning two rows (16-31), and every tile
+# of the receive-y half, to das_adj
+ROWS = C.scaled(C.get("cfg1"), nx=24, ny=3, nz=128, n_scat=300)
+
+
+@pytest.mark.parametrize("interp", ["quadratic", "lanczos4"])
+@pytest.mark.parametrize("half_index", [0, 1])
+def test_adjoint_tiles_along_rows(dev, interp, half_index):
+    say(f"case adjoint, tiles along grid rows (line-outer + general kernels), interp={interp} "
+        f"half={half_index}: vs untiled, oracle, frame blocks, per-tilt gradients")
+    flat, tiled, G = plans(ROWS, half_index, dev, interp)
+    rng = np.random.default_rng(31 + half_index)
+    nf = 5   # a frame block of 4 and a partial one
+    g = torch.from_numpy((rng.standard_normal((nf, 1, flat.n_pix)) +
+                          1j * rng.standard_normal((nf, 1, flat.n_pix))).astype(np.complex64)).to(dev)
+    a_f = flat.adjoint(g).cpu().numpy()
+    outs = {}
+    try:
+        for v, fb in ((0, 0), (4, 1), (4, 2), (4, 4), (2, 4)):
+            _lib.set_tuning(v, fb, 0, 0)
+            outs[(v, fb)] = tiled.adjoint(g).cpu().numpy()
+    finally:
+        _lib.set_tuning(0, 0, 0, 0)
+    for k, o in outs.items():
+        r = H.rel_l2(o, a_f)
+        say(f"V={k[0]} FB={k[1]}: rel-L2 vs untiled {r:.3e}")
+        assert r <= TOL, k
+    T = H.case(ROWS, half_index)[1]
+    kind, a = OKIND[interp]
+    ref = O.kernel_adjoint(g.cpu().numpy(), H.grid_f32(G), O.prepare(G, T),
+                           H.oracle_params(ROWS, ROWS.halves[half_index]), nf, kind, a)
+    r = H.rel_l2(outs[(0, 0)], ref)
+    say(f"tiled adjoint vs FP64 oracle: rel-L2 {r:.3e}")
+    assert r <= 1e-5
+    gt = torch.from_numpy((rng.standard_normal((2, flat.n_tx, flat.n_pix)) +
+                           1j * rng.standard_normal((2, flat.n_tx, flat.n_pix))).astype(np.complex64)).to(dev)
+    r = H.rel_l2(tiled.adjoint(gt).cpu().numpy(), flat.adjoint(gt).cpu().numpy())
+    say(f"per-tilt gradients, tiled vs untiled: rel-L2 {r:.3e}")
+    assert r <= TOL
+
+
+@pytest.mark.parametrize("kind", ["channel", "tx_channel"])
+def test_adjoint_tiles_along_rows_laws(dev, kind):
+    from tests.test_gpu_ablaw import laws, plans as law_plans
+    say(f"case adjoint, tiles along grid rows, law {kind}: tiled vs untiled vs FP64 oracle adjoint")
+    pkw, okw = laws(ROWS, kind, seed=23)
+    flat, tiled, G, T = law_plans(ROWS, 0, dev, "lanczos4", pkw)
+    rng = np.random.default_rng(29)
+    nf = 4
+    g = torch.from_numpy((rng.standard_normal((nf, 1, flat.n_pix)) +
+                          1j * rng.standard_normal((nf, 1, flat.n_pix))).astype(np.complex64)).to(dev)
+    a_f = flat.adjoint(g).cpu().numpy()
+    a_t = tiled.adjoint(g).cpu().numpy()
+    r = H.rel_l2(a_t, a_f)
+    say(f"tiled vs untiled: rel-L2 {r:.3e}")
+    assert r <= TOL
+    kind_o, a = OKIND["lanczos4"]
+    ref = O.kernel_adjoint(g.cpu().numpy(), H.grid_f32(G), O.prepare(G, T),
+                           H.oracle_params(ROWS, ROWS.halves[0]), nf, kind_o, a, **okw)
+    r = H.rel_l2(a_t, ref)
     say(f"tiled adjoint vs FP64 oracle: rel-L2 {r:.3e}")
     assert r <= 1e-5
 

@@ -36,11 +36,14 @@ def _ok(r, what, passes=None):
 # the launch layouts: forward FP32 one frame (the drop-in), forward FP32 and
 # FP16 frame blocks of 8 (ensembles), the serving batch kernels (FP32 one
 # frame at 32 KiB, FP16 blocks of 4 at 48 KiB), the adjoint (4 frames x 4
-# voxels per lane, the default; 2 x 2 for fewer frames; one voxel per lane)
+# voxels per lane, the default; 2 x 2 for fewer frames; one voxel per lane;
+# the line-outer kernel's windows: one tilt per window, 192-sample rows, the
+# layout the 16 KiB budget gives)
 LAYOUTS = {"fwd_f32": dict(), "fwd_f32_fb8": dict(fb=8), "fwd_f16_fb8": dict(sbytes=4, fb=8),
            "batch_f16_fb4": dict(sbytes=4, fb=4, lds_kib=48),
            "adjoint": dict(kernel="adjoint", fb=4, v=4), "adjoint_fb2_v2": dict(kernel="adjoint", fb=2, v=2),
-           "adjoint_v1": dict(kernel="adjoint")}
+           "adjoint_v1": dict(kernel="adjoint"),
+           "adjoint_line_outer": dict(kernel="adjoint", fb=4, v=4, lds_kib=16)}
 
 
 @pytest.mark.parametrize("interp", ["quadratic", "lanczos", "lanczos3", "lanczos4"])
@@ -65,11 +68,12 @@ def _sample_blocks(nblk, ndc, n_rand, seed):
     return np.unique(b)
 
 
-@pytest.mark.parametrize("layout", ["fwd_f32_fb8", "fwd_f16_fb8", "batch_f16_fb4", "adjoint"])
+@pytest.mark.parametrize("layout", ["fwd_f32_fb8", "fwd_f16_fb8", "batch_f16_fb4", "adjoint",
+                                    "adjoint_line_outer"])
 def test_cfg1_other_layouts(layout):
     cfg = C.get("cfg1")
     kw = LAYOUTS[layout]
-    if layout == "adjoint":      # 64 depths x 16 columns
+    if layout.startswith("adjoint"):      # 64 depths x 16 columns
         nblk, ndc = (128 * 128 // 16) * 4, 4
     else:                        # 16 depths x 16 columns
         nblk, ndc = (128 * 128 // 16) * 16, 16
@@ -78,7 +82,7 @@ def test_cfg1_other_layouts(layout):
             f"cfg1 half {half} {layout} (sampled workgroups)")
 
 
-@pytest.mark.parametrize("layout", ["fwd_f32", "fwd_f32_fb8", "adjoint", "adjoint_fb2_v2"])
+@pytest.mark.parametrize("layout", ["fwd_f32", "fwd_f32_fb8", "adjoint", "adjoint_fb2_v2", "adjoint_line_outer"])
 def test_cfg4_full(layout):
     """96x96x192 voxels, 16 tilts x 96 lines, 1024 samples, Lanczos a=3."""
     _ok(W.check(C.get("cfg4"), 0, **LAYOUTS[layout]), f"cfg4 {layout}")
