@@ -15,8 +15,9 @@
 //     the grid is depth-fastest) and FB frames.  It walks tilt -> line; for
 //     each (tilt, line) the FB I/Q traces are staged once in LDS and every
 //     lane gathers its 2a interpolation taps from LDS.
-//   * Geometry is FP64 (full-rate on CDNA4 vector ALUs): transmit delay per
-//     (voxel, tilt), receive distance per update.  The sample position u and
+//   * Geometry is FP64 (MI355X's vector FP64 peak, 78.6 TFLOP/s, is half its
+//     FP32 peak, so it is kept out of the per-update work): transmit delay per
+//     (voxel, tilt), receive distance per (voxel, line).  The sample position u and
 //     the demodulation phase are range-reduced in FP64 (u - floor(u),
 //     p - rint(p) in cycles, with the 4 pi fd z / c term cancelled
 //     analytically) before anything is rounded to FP32, so FP32 carries only
@@ -431,13 +432,14 @@ __device__ __forceinline__ void pixel_law_range(const KArgs& a, const double (&l
   hi = uni(fmaxf(fmaxf(b0.y, b1.y), fmaxf(b2.y, b3.y)));
 }
 
+// The same from the line's receive bounds (r_lo > r_hi: the aperture misses
+// the box).
 template <int LO, int HI, int E>
-__device__ __forceinline__ bool line_window(const KArgs& a, const TileBox& b, double e_lat,
-                                            double ezd, double uabc, float bu_lo, float bu_hi,
-                                            bool ab_tc, int t0, int nt, int c, int lane, int& s0,
-                                            int& whi, float px_lo = 0.0f, float px_hi = 0.0f) {
-  float r_lo, r_hi;
-  if (!rca_tile::rx_bounds(a.kfs, b, e_lat, ezd, uabc, r_lo, r_hi)) return false;
+__device__ __forceinline__ bool line_window_r(const KArgs& a, float r_lo, float r_hi, float bu_lo,
+                                              float bu_hi, bool ab_tc, int t0, int nt, int c,
+                                              int lane, int& s0, int& whi, float px_lo = 0.0f,
+                                              float px_hi = 0.0f) {
+  if (!(r_lo <= r_hi)) return false;
   if (!(px_lo <= px_hi)) return false;   // PIXEL_CHANNEL law: no active voxel
   r_lo += px_lo;
   r_hi += px_hi;
@@ -457,6 +459,44 @@ __device__ __forceinline__ bool line_window(const KArgs& a, const TileBox& b, do
   whi = uni(w1);
   s0 = uni(w0);
   return true;
+}
+
+template <int LO, int HI, int E>
+__device__ __forceinline__ bool line_window(const KArgs& a, const TileBox& b, double e_lat,
+                                            double ezd, double uabc, float bu_lo, float bu_hi,
+                                            bool ab_tc, int t0, int nt, int c, int lane, int& s0,
+                                            int& whi, float px_lo = 0.0f, float px_hi = 0.0f) {
+  float r_lo, r_hi;
+  if (!rca_tile::rx_bounds(a.kfs, b, e_lat, ezd, uabc, r_lo, r_hi)) return false;
+  return line_window_r<LO, HI, E>(a, r_lo, r_hi, bu_lo, bu_hi, ab_tc, t0, nt, c, lane, s0, whi,
+                                  px_lo, px_hi);
+}
+
+// Receive sample-offset range of line c over the workgroup's box (r_lo >
+// r_hi when the line's aperture misses the box, or for c = -1).  The kernels
+// evaluate it for 64 lines at a time, one line per lane, and read a line's
+// bounds back with readlane, so the bounds' FP64 square roots run once per
+// 64 lines instead of on every lane for every line.
+struct LineBlock {
+  float r_lo, r_hi;
+};
+
+__device__ __forceinline__ LineBlock line_block(const KArgs& a, const TileBox& b, bool rx, int c) {
+  LineBlock L{INFINITY, -INFINITY};
+  if (c >= 0) {
+    const double ab = a.ab_mode == RCA_ABLAW_CHANNEL ? (double)a.ab[c] : 0.0;
+    float lo, hi;
+    if (rca_tile::rx_bounds(a.kfs, b, rx ? a.ex[c] : a.ey[c], a.ez[c], a.fs * ab, lo, hi)) {
+      L.r_lo = lo;
+      L.r_hi = hi;
+    }
+  }
+  return L;
+}
+
+// Lane j's value of x (j uniform), in a scalar register.
+__device__ __forceinline__ float read_lane(float x, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
 }
 
 // Forward DAS of one workgroup: 256 threads x V voxels, frames
@@ -551,7 +591,10 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
       for (int fi = 0; fi < FB; ++fi) accT[v][fi] = make_float2(0.0f, 0.0f);
     }
 
+    LineBlock lb;
     for (int c = 0; c < a.n_ch; ++c) {
+      if ((c & 63) == 0) lb = line_block(a, box, rx, c + lane < a.n_ch ? c + lane : -1);
+      const int cl = c & 63;
       const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
       const double e_lat = rx ? exd : eyd;
       double uabc = 0.0, pabc = 0.0;
@@ -580,8 +623,8 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
         ++line_no;
       }
       int s0, whi;
-      if (line_window<LO, HI, E>(a, box, e_lat, ezd, uabc, bu_lo, bu_hi, ab_tc, t0, nt, c, lane, s0,
-                                 whi, px_lo, px_hi)) {
+      if (line_window_r<LO, HI, E>(a, read_lane(lb.r_lo, cl), read_lane(lb.r_hi, cl), bu_lo, bu_hi,
+                                   ab_tc, t0, nt, c, lane, s0, whi, px_lo, px_hi)) {
         const int npass = pass_count(s0, whi, wcap, step);
 
         for (int k = 0; k < npass; ++k) {
@@ -1106,11 +1149,15 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
     float bu_lo, bu_hi;
     chunk_tx_bounds(a, box, t0, nt, lane, rx, bu_lo, bu_hi);
 
+    LineBlock lb;
     for (int ci = 0; ci < a.n_ch; ++ci) {
       // workgroups start at different lines, so the tiles resident at one
       // time add to different traces instead of contending for the same
       // samples in L2
       const int c = (ci + c_first) % a.n_ch;
+      if ((ci & 63) == 0)
+        lb = line_block(a, box, rx, ci + lane < a.n_ch ? (ci + lane + c_first) % a.n_ch : -1);
+      const int cl = ci & 63;
       const double exd = a.ex[c], eyd = a.ey[c], ezd = a.ez[c];
       const double e_lat = rx ? exd : eyd;
       double uabc = 0.0, pabc = 0.0;
@@ -1128,8 +1175,8 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
         ++line_no;
       }
       int s0, whi;
-      if (!line_window<LO, HI, 1>(a, box, e_lat, ezd, uabc, bu_lo, bu_hi, ab_tc, t0, nt, c, lane,
-                                  s0, whi, px_lo, px_hi))
+      if (!line_window_r<LO, HI, 1>(a, read_lane(lb.r_lo, cl), read_lane(lb.r_hi, cl), bu_lo, bu_hi,
+                                    ab_tc, t0, nt, c, lane, s0, whi, px_lo, px_hi))
         continue;
       const int npass = pass_count(s0, whi, wcap, step);
 
@@ -1309,8 +1356,13 @@ __device__ __forceinline__ void adj_block_lo(const KArgs& a, unsigned bx, unsign
                           a.vz[t0 + lane], a.d0[t0 + lane], bu_lo, bu_hi);
     lds_barrier();   // transmit terms published
 
+    LineBlock lb;
     for (int ci = 0; ci < a.n_ch; ++ci) {
       const int c = (ci + c_first) % a.n_ch;
+      if ((ci & 63) == 0)
+        lb = line_block(a, box, true, ci + lane < a.n_ch ? (ci + lane + c_first) % a.n_ch : -1);
+      const float r_lo = read_lane(lb.r_lo, ci & 63), r_hi = read_lane(lb.r_hi, ci & 63);
+      if (!(r_lo <= r_hi)) continue;
       const double exd = a.ex[c], ezd = a.ez[c];
       double uabc = 0.0, pabc = 0.0;
       if (a.ab_mode == RCA_ABLAW_CHANNEL) {
@@ -1318,8 +1370,6 @@ __device__ __forceinline__ void adj_block_lo(const KArgs& a, unsigned bx, unsign
         uabc = a.fs * ab;
         pabc = a.fd * ab;
       }
-      float r_lo, r_hi;
-      if (!rca_tile::rx_bounds(a.kfs, box, exd, ezd, uabc, r_lo, r_hi)) continue;
       // receive terms of the lane's voxels on this line (.cu:284-311, 371-373)
       bool inap[V];
       int ri[V];
@@ -1345,8 +1395,8 @@ __device__ __forceinline__ void adj_block_lo(const KArgs& a, unsigned bx, unsign
       for (int j = 0; j < nt; ++j) {
         const int t = t0 + j;
         int s0, whi;
-        if (!line_window<LO, HI, 1>(a, box, exd, ezd, uabc, __shfl(bu_lo, j, 64), __shfl(bu_hi, j, 64),
-                                    ab_tc, t, 1, c, lane, s0, whi))
+        if (!line_window_r<LO, HI, 1>(a, r_lo, r_hi, __shfl(bu_lo, j, 64), __shfl(bu_hi, j, 64),
+                                      ab_tc, t, 1, c, lane, s0, whi))
           continue;
         const int npass = pass_count(s0, whi, wcap, step);
         for (int k = 0; k < npass; ++k) {
