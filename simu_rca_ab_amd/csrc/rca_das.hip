@@ -148,16 +148,21 @@ template <> struct Interp<RCA_INTERP_LANCZOS4> { static constexpr int LO = 3, HI
 // (bfVirtualSourceGeneric_RCA.cu:79-82, 331-336 for W = 2).
 //
 // With e = d - 1/2 the taps pair up, q_{1-p}(e) = q_p(-e): the even and odd
-// parts of q_p are polynomials of degree 4 in u = e^2 (rca_interp_poly.h,
-// fitted near-minimax on |e| <= 1/2, max |error| 1.4e-7 in FP32), so a pair
-// of taps costs 10 FMAs and two adds: 46 VALU instructions for the 8 taps of
-// a = 4, against about 95 for the sine-identity form with one reciprocal per
-// update it replaces (same bar against the FP64 oracle).
-// Horner steps with the coefficient as a 32-bit literal of v_fmaak_f32
-// (x * y + K).  Left to itself the compiler keeps these loop-invariant
-// constants in SGPRs, and an FMA with an SGPR source issues at half the rate
-// of one with VGPR or literal sources (4.0 vs 2.4 cycles per wave64
-// instruction, measured); the leading coefficient is moved into a VGPR.
+// parts of q_p are polynomials in u = e^2 (rca_interp_poly.h, fitted
+// near-minimax on |e| <= 1/2; degree 3 for a = 4, max |error| 4.3e-7 in FP32,
+// degree 4 for the 6-tap kinds, 1.7e-7), stored in monic form lead * M(u).
+// M is evaluated by Horner steps whose coefficients are 32-bit literals: an
+// add (u + a_top, one register operand), then v_fmaak_f32 (p * u + a_j); the
+// leads fold into one multiply for the even part and one for e of the odd
+// part, and each tap is one FMA: q_{p,1-p} = fma(+-e*leadO, M_O, leadE*M_E).
+// A pair costs 2 adds, 2 (DEG - 1) FMAs, 2 multiplies and 2 FMAs: 36
+// instructions for the 8 taps of a = 4 (half of them with one register
+// operand, which issue at twice the rate of two- and three-operand ones on
+// gfx950), against 46 for the degree-4 Horner form with the leading
+// coefficient moved into a register and 95 for the sine-identity form with
+// one reciprocal per update.  Left to itself the compiler keeps the
+// loop-invariant constants in SGPRs, and an FMA with an SGPR source issues at
+// half rate: hence the literals, in inline asm.
 template <unsigned K>
 __device__ __forceinline__ float fma_lit(float x, float y) {
   float r;
@@ -165,9 +170,15 @@ __device__ __forceinline__ float fma_lit(float x, float y) {
   return r;
 }
 template <unsigned K>
-__device__ __forceinline__ float vgpr_lit() {
+__device__ __forceinline__ float add_lit(float x) {
   float r;
-  asm("v_mov_b32 %0, %1" : "=v"(r) : "i"(K));
+  asm("v_add_f32 %0, %1, %2" : "=v"(r) : "i"(K), "v"(x));
+  return r;
+}
+template <unsigned K>
+__device__ __forceinline__ float mul_lit(float x) {
+  float r;
+  asm("v_mul_f32 %0, %1, %2" : "=v"(r) : "i"(K), "v"(x));
   return r;
 }
 template <int KIND, int P, bool ODD>
@@ -176,24 +187,32 @@ constexpr unsigned coef_bits(int j) {
                                           : InterpPoly<KIND>::kEven[P - 1][j]);
 }
 template <int KIND, int P, bool ODD>
-__device__ __forceinline__ float horner4(float u) {
-  constexpr unsigned c0 = coef_bits<KIND, P, ODD>(0), c1 = coef_bits<KIND, P, ODD>(1),
-                     c2 = coef_bits<KIND, P, ODD>(2), c3 = coef_bits<KIND, P, ODD>(3),
-                     c4 = coef_bits<KIND, P, ODD>(4);
-  float p = fma_lit<c3>(u, vgpr_lit<c4>());
-  p = fma_lit<c2>(u, p);
-  p = fma_lit<c1>(u, p);
-  return fma_lit<c0>(u, p);
+constexpr unsigned lead_bits() {
+  return __builtin_bit_cast(unsigned, ODD ? InterpPoly<KIND>::kOddLead[P - 1]
+                                          : InterpPoly<KIND>::kEvenLead[P - 1]);
+}
+// p * u + a_J, then the lower coefficients down to a_0.
+template <int KIND, int P, bool ODD, int J>
+__device__ __forceinline__ float horner_tail(float u, float p) {
+  if constexpr (J < 0) return p;
+  else return horner_tail<KIND, P, ODD, J - 1>(u, fma_lit<coef_bits<KIND, P, ODD>(J)>(u, p));
+}
+// M(u) = u^D + a_{D-1} u^{D-1} + ... + a_0.
+template <int KIND, int P, bool ODD>
+__device__ __forceinline__ float horner_monic(float u) {
+  constexpr int D = InterpPoly<KIND>::kDeg;
+  return horner_tail<KIND, P, ODD, D - 2>(u, add_lit<coef_bits<KIND, P, ODD>(D - 1)>(u));
 }
 
 // q_p and q_{1-p} from the even and odd parts of pair p.
 template <int KIND, int P>
 __device__ __forceinline__ void weight_pair(float e, float u, float* q) {
   constexpr int LO = Interp<KIND>::LO;
-  const float pe = horner4<KIND, P, false>(u);
-  const float po = horner4<KIND, P, true>(u) * e;
-  q[P + LO] = pe + po;
-  q[1 - P + LO] = pe - po;
+  const float pe = mul_lit<lead_bits<KIND, P, false>()>(horner_monic<KIND, P, false>(u));
+  const float mo = horner_monic<KIND, P, true>(u);
+  const float eo = mul_lit<lead_bits<KIND, P, true>()>(e);
+  q[P + LO] = fmaf(eo, mo, pe);
+  q[1 - P + LO] = fmaf(-eo, mo, pe);
   if constexpr (P > 1) weight_pair<KIND, P - 1>(e, u, q);
 }
 
@@ -206,7 +225,7 @@ __device__ __forceinline__ void interp_weights(float d, float* q) {
     q[1] = -(d - 1.0f) * (d + 1.0f);
     q[2] = d * (d + 1.0f) * 0.5f;
   } else {
-    static_assert(HI == LO + 1 && kPolyDeg == 4, "taps pair as p and 1 - p; degree-4 parts");
+    static_assert(HI == LO + 1 && InterpPoly<KIND>::kDeg >= 2, "taps pair as p and 1 - p");
     const float e = d - 0.5f;
     weight_pair<KIND, HI>(e, e * e, q);
   }
