@@ -352,10 +352,15 @@ __device__ __forceinline__ TileBox tile_voxels(const KArgs& a, unsigned bx, int 
   return b;
 }
 
+// Transmit sample offset stored for a (voxel, tilt) outside the transmit
+// cone when OFF: far beyond any trace, so every sample position built on it
+// fails the forward's range test (real offsets stay within +-2^30 samples).
+constexpr int kTxOff = 1 << 30;
+
 // Transmit terms of tilts [t0, t0 + nt) for the lane's voxels into the LDS
 // cache ([j][v][lane]); bit j of act[v] = tilt t0 + j insonifies voxel v
 // (.cu:191-214).
-template <int V>
+template <int V, bool OFF = false>
 __device__ __forceinline__ void chunk_tx(const KArgs& a, int t0, int nt, bool rx, const double* pl,
                                          const double* pt, const double* pz, const bool* valid,
                                          TxTerm* txc, int tid, unsigned* act) {
@@ -377,9 +382,13 @@ __device__ __forceinline__ void chunk_tx(const KArgs& a, int t0, int nt, bool rx
       TxTerm x;
       split_pos(fma(a.kfs, tt, a.ut0), x.bi, x.bf);
       x.et = make_float2(cs, sn);
-      txc[(j * V + v) * kThreads + tid] = x;
+      // (stored before the cone test unless it may change the term: the
+      // term's registers then need not live across the test)
+      if constexpr (!OFF) txc[(j * V + v) * kThreads + tid] = x;
       const float fx = (float)(rx ? pl[v] : pt[v]), fy = (float)(rx ? pt[v] : pl[v]);
       if (!a.mask_tx || tx_mask(a, vxf, vyf, vzf, fx, fy, (float)pz[v])) act[v] |= 1u << j;
+      else if constexpr (OFF) x.bi = kTxOff;
+      if constexpr (OFF) txc[(j * V + v) * kThreads + tid] = x;
     }
   }
 }
@@ -574,6 +583,10 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   const bool rx = a.receive_x != 0;
   constexpr bool ab_tc = LAW == kLawTxLine;
   constexpr bool ab_px = LAW == kLawPixel;
+  // single-frame kernels mark uninsonified (voxel, tilt) terms with kTxOff
+  // instead of testing the tilt's bit per update (the frame-block kernels
+  // test the bit: the marking there cost two VGPRs past the 4-wave limit)
+  constexpr bool kOff = FB == 1;
   bool ok;   // scratch for voxel_of at the stores
 
   // receive-axis and transmit-axis coordinates, depth, aperture half-width
@@ -597,7 +610,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
   for (int t0 = 0; t0 < a.n_tx; t0 += tcn) {
     const int nt = min(tcn, a.n_tx - t0);
     unsigned act[V];
-    chunk_tx<V>(a, t0, nt, rx, pl, pt, pz, valid, txc, tid, act);
+    chunk_tx<V, kOff>(a, t0, nt, rx, pl, pt, pz, valid, txc, tid, act);
     float bu_lo, bu_hi;
     chunk_tx_bounds(a, box, t0, nt, lane, rx, bu_lo, bu_hi);
 
@@ -683,6 +696,11 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
             }
           }
           __syncthreads();
+          // the updates this pass takes: first tap base = m - LO in [cb, hi)
+          // (the range test m in [LO, ns - 1 - HI) of .cu:323 included), as
+          // t = base - cb < span in unsigned arithmetic
+          const int hi = last ? ns - NT : min(ns - NT, cb + step);
+          const unsigned span = hi > cb ? (unsigned)(hi - cb) : 0u;
 
 #pragma unroll
           for (int v = 0; v < V; ++v) {
@@ -707,19 +725,30 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
             float sn, cs;
             sincos2pi((float)(ph - rint(ph)), sn, cs);
             const float er = w * cs, ei = w * sn;
+            const unsigned rio = (unsigned)(ri - LO - cb);
             const TxTerm* tv = txc + v * kThreads + tid;
             const S* rows = tr;
 #pragma unroll 2
             for (int j = 0; j < nt; ++j, tv += V * kThreads, rows += FB * wcap) {
-              if (!((act[v] >> j) & 1u)) continue;
+              if (!kOff && !((act[v] >> j) & 1u)) continue;   // .cu:191-214
+              // .cu:310-319: u = (bi + bf) + (ri + rf) [+ the per-tilt law],
+              // m = floor(u), t = m - LO - cb
               const TxTerm x = *tv;
-              int m;
-              float d;
-              if (ab_tc) sample_pos(x.bi + ri + s_uab_i[j], x.bf + rf + s_uab_f[j], m, d);
-              else sample_pos(x.bi + ri, x.bf + rf, m, d);                 // .cu:310-319
-              if (!(m >= LO && m < ns - 1 - HI)) continue;                  // .cu:323
-              const int base = m - LO;
-              if (!rca_tile::in_pass(base, k, cb, last, step)) continue;
+              // the phasor is used only past the range test; this empty use
+              // keeps the term one 16-byte LDS read (split in two 8-byte
+              // reads at a 16-byte lane stride, it read with 2-way bank
+              // conflicts)
+              __asm__ volatile("" ::"v"(x.et.x), "v"(x.et.y));
+              float f = x.bf + rf;
+              unsigned tb = (unsigned)x.bi + rio;
+              if (ab_tc) {
+                f += s_uab_f[j];
+                tb += (unsigned)s_uab_i[j];
+              }
+              const float fl = floorf(f);
+              const unsigned t = tb + (unsigned)(int)fl;
+              if (t >= span) continue;
+              const float d = f - fl;
               float q[NT];
               interp_weights<KIND>(d, q);
               float pr = er * x.et.x - ei * x.et.y;
@@ -732,7 +761,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
               }
               // the bounds put every tap inside the row; the clamp only keeps
               // LDS reads inside it
-              const S* trv = rows + min(max(base - cb, 0), wcap - NT);
+              const S* trv = rows + min(t, (unsigned)(wcap - NT));
 #pragma unroll
               for (int fi = 0; fi < FB; ++fi) {
                 if (fi < nf) {
@@ -1027,7 +1056,7 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
     else sample_pos(x.bi + ri, x.bf + rf, m, d);                 // .cu:310-319
     if (!(m >= LO && m < ns - 1 - HI)) continue;                  // .cu:323
     const int base = m - LO;
-    if (!rca_tile::in_pass(base, k, cb, last, step)) continue;
+    if (!rca_tile::in_pass(base, cb, last, step)) continue;
     float q[NT];
     interp_weights<KIND>(d, q);
     float pr = er * x.et.x - ei * x.et.y;

@@ -139,10 +139,12 @@ __host__ __device__ __forceinline__ void window_from(int ns, int LO, int HI, int
 
 // Passes over a window longer than the rows: rows hold wcap samples; pass k
 // stages [s0 + k*step, min(s0 + k*step + wcap, whi)) and takes the updates
-// whose first tap lies in [s0 + k*step, s0 + (k+1)*step) (the first pass also
-// those below, the last pass also those above).  step is at least one 16-byte
-// chunk so the pass loop always ends (rows shorter than NT + E only occur with
-// traces too short for any tap).
+// whose first tap lies in [s0 + k*step, s0 + (k+1)*step) (the last pass also
+// those above).  An update whose first tap lies below s0 belongs to no pass:
+// the window bounds leave none (the host emulation of the launch layouts
+// counts any as a violation).  step is at least one 16-byte chunk so the pass
+// loop always ends (rows shorter than NT + E only occur with traces too short
+// for any tap).
 __host__ __device__ __forceinline__ int pass_step(int wcap, int NT, int E) {
   const int s = (wcap - NT + 1) / E * E;
   return s > E ? s : E;
@@ -150,8 +152,8 @@ __host__ __device__ __forceinline__ int pass_step(int wcap, int NT, int E) {
 __host__ __device__ __forceinline__ int pass_count(int s0, int whi, int wcap, int step) {
   return whi - s0 > wcap ? 1 + (whi - s0 - wcap + step - 1) / step : 1;
 }
-__host__ __device__ __forceinline__ bool in_pass(int base, int k, int cb, bool last, int step) {
-  return !((k > 0 && base < cb) || (!last && base >= cb + step));
+__host__ __device__ __forceinline__ bool in_pass(int base, int cb, bool last, int step) {
+  return base >= cb && (last || base < cb + step);
 }
 
 // x = i + f with i = floor(x) and f = x - i in [0, 1], computed in FP64.

@@ -258,7 +258,7 @@ int rca_window_check(const WinCase* wc, WinStats* st) {
               for (int k = 0; k < npass; ++k) {
                 const int cb = s0 + k * step;
                 const bool last = cb + wcap >= whi;
-                if (in_pass(base, k, cb, last, step)) { ++taken; kk = k; }
+                if (in_pass(base, cb, last, step)) { ++taken; kk = k; }
               }
               int kind = 0, cb = 0, end = 0;
               if (taken != 1) {
