@@ -269,7 +269,8 @@ def run_cfg2(dev):
 
 def run_adjoint(dev, reps=3):
     """DAS forward and adjoint (the backward of the beamformer) on config 4:
-    96+96 elements, 16 tilts, 96x96x192 voxels, 8 frames, Lanczos a=3."""
+    96+96 elements, 16 tilts, 96x96x192 voxels, 8 frames, Lanczos a=3; the
+    adjoint with float atomics and the deterministic (fixed-point) one."""
     import torch
     from simu_rca_ab_amd import configs as C, synth
     cfg = C.get("cfg4")
@@ -281,7 +282,8 @@ def run_adjoint(dev, reps=3):
     gout = torch.randn((cfg.n_frame, 1, plan.n_pix), dtype=torch.complex64, device=dev, generator=g)
     res = {}
     for name, fn in (("forward", lambda: plan.forward(iq, want_mask=False)),
-                     ("adjoint", lambda: plan.adjoint(gout))):
+                     ("adjoint", lambda: plan.adjoint(gout)),
+                     ("adjoint_deterministic", lambda: plan.adjoint(gout, deterministic=True))):
         fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()

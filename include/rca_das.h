@@ -152,6 +152,24 @@ int rca_adjoint(const rca_das_desc* desc, const void* grad_out, int grad_tx, int
                 void* grad_iq, int flags, hipStream_t stream);
 
 /*
+ * Deterministic adjoint (no reference counterpart; for training runs that
+ * must reproduce bit for bit): the same A^H grad_out as rca_adjoint, with
+ * the tiles' partial sums added as 64-bit fixed-point integers (2^-S quanta,
+ * S set from n_pix and the largest component of grad_out so no sum can
+ * overflow; about 1e-12 of that component) in `workspace`, then converted to
+ * FP32 into grad_iq (overwritten, or added to with RCA_FLAG_ACCUMULATE).
+ * Integer addition does not depend on the order in which tiles finish, so
+ * repeated calls on the same inputs give identical grad_iq.  workspace:
+ * device memory, 256-byte aligned, of rca_adjoint_det_workspace_bytes bytes
+ * (256 + 16 per complex gradient sample); the call allocates nothing.  A
+ * non-finite grad_out gives NaN throughout grad_iq.
+ */
+size_t rca_adjoint_det_workspace_bytes(const rca_das_desc* desc, int n_frame);
+int rca_adjoint_det(const rca_das_desc* desc, const void* grad_out, int grad_tx, int n_frame,
+                    void* grad_iq, int flags, void* workspace, size_t workspace_bytes,
+                    hipStream_t stream);
+
+/*
  * Batched forward for serving (no reference counterpart; the reference makes
  * one synchronous call per chunk, bfVirtualSourceGeneric_RCA.m:664-711).
  * A batch is a list of entries; each entry beamforms frames

@@ -24,7 +24,8 @@ EXPORTS = ("bfVirtualSourceGeneric_RCA", "rca_forward", "rca_adjoint", "rca_last
            "rca_batch_launch", "rca_batch_frame_block", "rca_svd_workspace_bytes", "rca_svd_filter", "rca_sim_txfield",
            "rca_sim_rxfield", "rca_sim_receive", "rca_sim_spectra_to_raw", "rca_prepare_geometry",
            "rca_plan_create", "rca_plan_desc", "rca_plan_destroy", "rca_probe_valu",
-           "rca_batch_launch_noalloc", "rca_quantize_fp16")
+           "rca_batch_launch_noalloc", "rca_quantize_fp16", "rca_adjoint_det_workspace_bytes",
+           "rca_adjoint_det")
 
 
 class RcaDasDesc(ctypes.Structure):
@@ -81,6 +82,9 @@ def load():
     dp = ctypes.POINTER(RcaDasDesc)
     lib.rca_forward.argtypes = [dp, vp, i, vp, i, vp, vp, i, vp]
     lib.rca_adjoint.argtypes = [dp, vp, i, i, vp, i, vp]
+    lib.rca_adjoint_det_workspace_bytes.argtypes = [dp, i]
+    lib.rca_adjoint_det_workspace_bytes.restype = ctypes.c_size_t
+    lib.rca_adjoint_det.argtypes = [dp, vp, i, i, vp, i, vp, ctypes.c_size_t, vp]
     lib.rca_set_tuning.argtypes = [i, i, i, i]
     lib.bfVirtualSourceGeneric_RCA.argtypes = [vp] * 13 + [i, f, f, f, f, f, f, f, f] + [i] * 7 + [vp]
     ll, sz = ctypes.c_longlong, ctypes.c_size_t
@@ -99,7 +103,8 @@ def load():
     lib.rca_sim_spectra_to_raw.argtypes = [vp, i, i, i, i, vp, vp]
     lib.rca_probe_valu.argtypes = [i, vp, i, i, vp, ctypes.POINTER(ll)]
     lib.rca_quantize_fp16.argtypes = [vp, ll, i, vp, vp, vp]
-    for name in ("rca_quantize_fp16", "rca_probe_valu", "rca_forward", "rca_adjoint", "rca_set_tuning", "bfVirtualSourceGeneric_RCA",
+    for name in ("rca_quantize_fp16", "rca_probe_valu", "rca_forward", "rca_adjoint", "rca_adjoint_det",
+                 "rca_set_tuning", "bfVirtualSourceGeneric_RCA",
                  "rca_svd_filter", "rca_sim_txfield", "rca_sim_rxfield", "rca_sim_receive",
                  "rca_sim_spectra_to_raw", "rca_prepare_geometry", "rca_plan_create",
                  "rca_plan_destroy"):

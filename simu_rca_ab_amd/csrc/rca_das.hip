@@ -117,6 +117,9 @@ struct KArgs {
   int depth_run, wcap;   // voxel tiling hint; LDS samples per row
   int tc;                // tilts per chunk of the forward kernel
   int adj_lo;            // adjoint: tiles whose columns share the transmit terms go to das_adj_lo
+  // deterministic adjoint: 64-bit fixed-point sums instead of float atomics
+  unsigned long long* gdet;   // [2 x gradient samples], or null
+  const unsigned* gdet_max;   // max over grad_out of max(|re|, |im|), as float bits
 };
 
 // ----------------------------------------------------------------------------
@@ -1009,6 +1012,34 @@ __device__ __forceinline__ void lds_barrier() {
   __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// Deterministic adjoint.  Float atomics add the tiles' partial sums in
+// whatever order the tiles finish; 64-bit integer atomics give the same sum
+// in any order.  A partial sum v is added as llrint(v * 2^S), with S set so
+// that every sum over the voxels of one gradient sample stays below 2^62: a
+// voxel adds |q w (g.re cos + g.im sin)| <= 2 gmax (weights |q| <= 1, w <= 1),
+// so the bound is 2.25 n_pix gmax (gmax = max over grad_out of max(|re|,
+// |im|)).  The quantum is 2^-S ~ 2.25 n_pix gmax / 2^62, about 1e-12 gmax at
+// config 4.  S = 62 - e with 2^(e-1) <= bound < 2^e; scale 1 when gmax is 0.
+__host__ __device__ __forceinline__ double det_scale(float gmax, long long n_pix, bool inverse) {
+  if (!(gmax > 0.0f) || n_pix < 1) return 1.0;
+  const int e = ilogb(2.25 * (double)n_pix * (double)gmax) + 1;
+  return ldexp(1.0, inverse ? e - 62 : 62 - e);
+}
+
+// Adds one flushed partial sum (re, im) to gradient sample i.
+__device__ __forceinline__ void flush_add(const KArgs& a, double dsc, long long i, float vr,
+                                          float vi) {
+  if (a.gdet) {
+    unsigned long long* w = a.gdet + 2 * i;
+    if (vr != 0.0f) atomicAdd(w, (unsigned long long)(long long)llrint((double)vr * dsc));
+    if (vi != 0.0f) atomicAdd(w + 1, (unsigned long long)(long long)llrint((double)vi * dsc));
+  } else {
+    float* dst = reinterpret_cast<float*>(a.gin + i);
+    if (vr != 0.0f) unsafeAtomicAdd(dst, vr);
+    if (vi != 0.0f) unsafeAtomicAdd(dst + 1, vi);
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Adjoint kernel (the transpose of fwd_block): 256 lanes, one voxel each,
 // frames [by*FB, by*FB + FB).  Same depth tiles (one column per wave: the 64
@@ -1151,6 +1182,8 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  // fixed-point scale of the deterministic flush (unused otherwise)
+  const double dsc = a.gdet ? det_scale(__uint_as_float(*a.gdet_max), a.n_pix, false) : 0.0;
   const int f0 = by * FB;
   const int nf = min(FB, a.n_frame - f0);
   const int ns = a.n_sample;
@@ -1285,14 +1318,10 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
         if (rj < nt && rfi < nf) {
           const float2* r = reinterpret_cast<const float2*>(rows + (size_t)row * 2 * wcap);
           const size_t wr2 = wrows / 2;   // complex samples per wave
-          float* dst = reinterpret_cast<float*>(
-              a.gin + (((long long)(f0 + rfi) * a.n_tx + t0 + rj) * a.n_ch + c) * ns + cb);
+          const long long i0 = (((long long)(f0 + rfi) * a.n_tx + t0 + rj) * a.n_ch + c) * ns + cb;
           for (int i = li; i < len; i += G) {
             const float2 v0 = r[i], v1 = r[wr2 + i], v2 = r[2 * wr2 + i], v3 = r[3 * wr2 + i];
-            const float vr = v0.x + v1.x + v2.x + v3.x;
-            const float vi = v0.y + v1.y + v2.y + v3.y;
-            if (vr != 0.0f) unsafeAtomicAdd(dst + 2 * i, vr);
-            if (vi != 0.0f) unsafeAtomicAdd(dst + 2 * i + 1, vi);
+            flush_add(a, dsc, i0 + i, v0.x + v1.x + v2.x + v3.x, v0.y + v1.y + v2.y + v3.y);
           }
         }
       }
@@ -1328,6 +1357,8 @@ __device__ __forceinline__ void adj_block_lo(const KArgs& a, unsigned bx, unsign
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  // fixed-point scale of the deterministic flush (unused otherwise)
+  const double dsc = a.gdet ? det_scale(__uint_as_float(*a.gdet_max), a.n_pix, false) : 0.0;
   const int f0 = by * FB;
   const int nf = min(FB, a.n_frame - f0);
   const int ns = a.n_sample;
@@ -1485,14 +1516,10 @@ __device__ __forceinline__ void adj_block_lo(const KArgs& a, unsigned bx, unsign
           if (row < nf) {
             const float2* r = reinterpret_cast<const float2*>(rows + (size_t)row * 2 * wcap);
             const size_t wr2 = wrows / 2;
-            float* dst = reinterpret_cast<float*>(
-                a.gin + (((long long)(f0 + row) * a.n_tx + t) * a.n_ch + c) * ns + cb);
+            const long long i0 = (((long long)(f0 + row) * a.n_tx + t) * a.n_ch + c) * ns + cb;
             for (int i = li; i < len; i += G) {
               const float2 v0 = r[i], v1 = r[wr2 + i], v2 = r[2 * wr2 + i], v3 = r[3 * wr2 + i];
-              const float vr = v0.x + v1.x + v2.x + v3.x;
-              const float vi = v0.y + v1.y + v2.y + v3.y;
-              if (vr != 0.0f) unsafeAtomicAdd(dst + 2 * i, vr);
-              if (vi != 0.0f) unsafeAtomicAdd(dst + 2 * i + 1, vi);
+              flush_add(a, dsc, i0 + i, v0.x + v1.x + v2.x + v3.x, v0.y + v1.y + v2.y + v3.y);
             }
           }
         }
@@ -1517,6 +1544,33 @@ __global__ void __launch_bounds__(kThreads) das_adj(const KArgs a) {
 __global__ void fill_zero(float* __restrict__ p, long long n) {
   const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   for (long long i = i0; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+
+// Deterministic adjoint, before the scatter: *out = max |x[i]| over n floats
+// (the re and im parts of grad_out), as float bits (*out starts at 0; the
+// bits of non-negative floats order like the floats; a NaN wins).
+__global__ void abs_max(const float* __restrict__ x, long long n, unsigned* __restrict__ out) {
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned m = 0u;
+  for (long long i = i0; i < n; i += (long long)gridDim.x * blockDim.x)
+    m = max(m, __float_as_uint(fabsf(x[i])));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
+// Deterministic adjoint, after the scatter: the fixed-point sums back to FP32,
+// stored or added to grad_iq (a non-finite grad_out gives NaN throughout).
+__global__ void det_to_float(const unsigned long long* __restrict__ w, const unsigned* __restrict__ gmax,
+                             float* __restrict__ out, long long n, long long n_pix, int accumulate) {
+  const float gm = __uint_as_float(*gmax);
+  const double inv = det_scale(gm, n_pix, true);
+  const bool finite = gm <= 3.4e38f;
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long long i = i0; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = finite ? (float)((double)(long long)w[i] * inv) : __int_as_float(0x7fc00000);
+    out[i] = accumulate ? out[i] + v : v;
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -1722,6 +1776,73 @@ int launch_adj_k(const KArgs& a, int FB, int V, hipStream_t st) {
   return fail(RCA_ERR_UNSUPPORTED, "no adjoint kernel for FB=%d V=%d", FB, V);
 }
 
+// Adjoint launch shared by rca_adjoint (float atomics) and rca_adjoint_det
+// (ws non-null: fixed-point sums in ws, then converted into grad_iq).
+constexpr size_t kDetHeader = 256;
+
+int adjoint_run(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_frame,
+                void* grad_iq, int flags, void* ws, hipStream_t stream) {
+  g_err.clear();
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!grad_out || !grad_iq) return fail(RCA_ERR_INVALID, "grad pointers are NULL");
+  if (grad_tx != 1 && grad_tx != d->n_tx)
+    return fail(RCA_ERR_INVALID, "grad_tx must be 1 or nTx (got %d)", grad_tx);
+  if (n_frame < 0) return fail(RCA_ERR_INVALID, "nFrame < 0");
+  if (flags & ~RCA_FLAG_ACCUMULATE) return fail(RCA_ERR_INVALID, "unknown flags 0x%x", flags);
+  if (n_frame == 0) return RCA_OK;
+  KArgs a = make_args(d, n_frame);
+  a.gout = reinterpret_cast<const float2*>(grad_out);
+  a.gin = reinterpret_cast<float2*>(grad_iq);
+  a.gout_tx = grad_tx;
+  a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
+  // frames per workgroup and voxels per lane: FB = 4 with V = 4 measured
+  // fastest on a 96x96x192 volume (8 frames); fewer frames take V = 2, which
+  // keeps the kernel at 3 waves per SIMD
+  const int tfb = g_tune_fb.load(), tv = g_tune_v.load();
+  int fb = tfb ? tfb : 4;
+  while (fb > 1 && fb > n_frame) fb >>= 1;
+  int v = tv ? tv : (fb == 4 ? 4 : 2);
+  if (fb == 8 && v > 2) v = 2;
+  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) fb = v = 1;   // the instantiated pixel-law shape
+  const long long n = 2LL * n_frame * d->n_tx * d->n_channel * d->n_sample;   // floats of grad_iq
+  hipError_t e;
+  if (ws) {
+    // fixed-point sums start at zero; grad_out's largest component sets their scale
+    unsigned char* w = reinterpret_cast<unsigned char*>(ws);
+    hipLaunchKernelGGL(fill_zero, dim3(4096), dim3(256), 0, stream, reinterpret_cast<float*>(w),
+                       (long long)((kDetHeader + 8 * (size_t)n) / 4));
+    a.gdet_max = reinterpret_cast<const unsigned*>(w);
+    a.gdet = reinterpret_cast<unsigned long long*>(w + kDetHeader);
+    const long long ng = 2LL * n_frame * grad_tx * d->n_pix;
+    hipLaunchKernelGGL(abs_max, dim3(2048), dim3(256), 0, stream,
+                       reinterpret_cast<const float*>(grad_out), ng, reinterpret_cast<unsigned*>(w));
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "deterministic adjoint set-up");
+  } else if (!a.accumulate) {
+    // every workgroup adds its windows to the traces: start from zero
+    hipLaunchKernelGGL(fill_zero, dim3(2048), dim3(256), 0, stream,
+                       reinterpret_cast<float*>(grad_iq), n);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "fill_zero launch");
+  }
+  if (d->n_pix > 0) {
+    switch (d->interp) {
+      case RCA_INTERP_QUADRATIC: rc = launch_adj_k<RCA_INTERP_QUADRATIC>(a, fb, v, stream); break;
+      case RCA_INTERP_LANCZOS_REF: rc = launch_adj_k<RCA_INTERP_LANCZOS_REF>(a, fb, v, stream); break;
+      case RCA_INTERP_LANCZOS3: rc = launch_adj_k<RCA_INTERP_LANCZOS3>(a, fb, v, stream); break;
+      case RCA_INTERP_LANCZOS4: rc = launch_adj_k<RCA_INTERP_LANCZOS4>(a, fb, v, stream); break;
+      default: return fail(RCA_ERR_INVALID, "unknown interpolation %d", d->interp);
+    }
+    if (rc) return rc;
+  }
+  if (ws) {
+    hipLaunchKernelGGL(det_to_float, dim3(4096), dim3(256), 0, stream,
+                       reinterpret_cast<const unsigned long long*>(a.gdet), a.gdet_max,
+                       reinterpret_cast<float*>(grad_iq), n, (long long)d->n_pix, a.accumulate);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "det_to_float launch");
+  }
+  return RCA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1803,45 +1924,25 @@ int rca_forward(const rca_das_desc* d, const void* iq, int iq_dtype, const float
 
 int rca_adjoint(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_frame,
                 void* grad_iq, int flags, hipStream_t stream) {
+  return adjoint_run(d, grad_out, grad_tx, n_frame, grad_iq, flags, nullptr, stream);
+}
+
+size_t rca_adjoint_det_workspace_bytes(const rca_das_desc* d, int n_frame) {
+  if (!d || n_frame < 0 || d->n_tx < 0 || d->n_channel < 0 || d->n_sample < 0) return 0;
+  return kDetHeader + 16 * (size_t)n_frame * d->n_tx * d->n_channel * d->n_sample;
+}
+
+int rca_adjoint_det(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_frame,
+                    void* grad_iq, int flags, void* workspace, size_t workspace_bytes,
+                    hipStream_t stream) {
   g_err.clear();
-  int rc = check_desc(d);
-  if (rc) return rc;
-  if (!grad_out || !grad_iq) return fail(RCA_ERR_INVALID, "grad pointers are NULL");
-  if (grad_tx != 1 && grad_tx != d->n_tx)
-    return fail(RCA_ERR_INVALID, "grad_tx must be 1 or nTx (got %d)", grad_tx);
-  if (n_frame < 0) return fail(RCA_ERR_INVALID, "nFrame < 0");
-  if (flags & ~RCA_FLAG_ACCUMULATE) return fail(RCA_ERR_INVALID, "unknown flags 0x%x", flags);
-  if (n_frame == 0) return RCA_OK;
-  KArgs a = make_args(d, n_frame);
-  a.gout = reinterpret_cast<const float2*>(grad_out);
-  a.gin = reinterpret_cast<float2*>(grad_iq);
-  a.gout_tx = grad_tx;
-  a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
-  // frames per workgroup and voxels per lane: FB = 4 with V = 4 measured
-  // fastest on a 96x96x192 volume (8 frames); fewer frames take V = 2, which
-  // keeps the kernel at 3 waves per SIMD
-  const int tfb = g_tune_fb.load(), tv = g_tune_v.load();
-  int fb = tfb ? tfb : 4;
-  while (fb > 1 && fb > n_frame) fb >>= 1;
-  int v = tv ? tv : (fb == 4 ? 4 : 2);
-  if (fb == 8 && v > 2) v = 2;
-  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) fb = v = 1;   // the instantiated pixel-law shape
-  // every workgroup adds its windows to the traces: start from zero
-  if (!a.accumulate) {
-    const long long n = 2LL * n_frame * d->n_tx * d->n_channel * d->n_sample;
-    hipLaunchKernelGGL(fill_zero, dim3(2048), dim3(256), 0, stream,
-                       reinterpret_cast<float*>(grad_iq), n);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "fill_zero launch");
-  }
-  if (d->n_pix == 0) return RCA_OK;
-  switch (d->interp) {
-    case RCA_INTERP_QUADRATIC: return launch_adj_k<RCA_INTERP_QUADRATIC>(a, fb, v, stream);
-    case RCA_INTERP_LANCZOS_REF: return launch_adj_k<RCA_INTERP_LANCZOS_REF>(a, fb, v, stream);
-    case RCA_INTERP_LANCZOS3: return launch_adj_k<RCA_INTERP_LANCZOS3>(a, fb, v, stream);
-    case RCA_INTERP_LANCZOS4: return launch_adj_k<RCA_INTERP_LANCZOS4>(a, fb, v, stream);
-  }
-  return fail(RCA_ERR_INVALID, "unknown interpolation %d", d->interp);
+  if (!workspace) return fail(RCA_ERR_INVALID, "workspace is NULL");
+  if (((uintptr_t)workspace & 255) != 0) return fail(RCA_ERR_INVALID, "workspace must be 256-byte aligned");
+  if (!d) return fail(RCA_ERR_INVALID, "desc is NULL");
+  const size_t need = rca_adjoint_det_workspace_bytes(d, n_frame);
+  if (workspace_bytes < need)
+    return fail(RCA_ERR_INVALID, "workspace of %zu bytes, %zu needed", workspace_bytes, need);
+  return adjoint_run(d, grad_out, grad_tx, n_frame, grad_iq, flags, workspace, stream);
 }
 
 int bfVirtualSourceGeneric_RCA(float* dataBf, float* mask, const float* data,

@@ -279,8 +279,13 @@ class RcaPlan:
         _lib.check(rc)
         return out, mask
 
-    def adjoint(self, grad_out, *, grad_iq=None, accumulate=False):
-        """grad_iq = A^H grad_out for grad_out complex64 [F, 1|T, P]."""
+    def adjoint(self, grad_out, *, grad_iq=None, accumulate=False, deterministic=False):
+        """grad_iq = A^H grad_out for grad_out complex64 [F, 1|T, P].
+
+        deterministic: sum the tiles' contributions as 64-bit fixed-point
+        integers in a workspace (``rca_adjoint_det``), so repeated calls give
+        bitwise identical gradients; otherwise float atomics (faster, with
+        run-to-run differences at the FP32 rounding level)."""
         if (grad_out.dtype != torch.complex64 or grad_out.dim() != 3 or not grad_out.is_contiguous()
                 or grad_out.shape[2] != self.n_pix or grad_out.shape[1] not in (1, self.n_tx)
                 or grad_out.device != self.device):
@@ -294,9 +299,16 @@ class RcaPlan:
         elif (grad_iq.dtype != torch.complex64 or tuple(grad_iq.shape) != shape
               or not grad_iq.is_contiguous() or grad_iq.device != self.device):
             raise ValueError(f"grad_iq must be contiguous complex64 {shape}")
-        rc = self.lib.rca_adjoint(ctypes.byref(self.desc), ctypes.c_void_p(grad_out.data_ptr()), gt,
-                                  nf, ctypes.c_void_p(grad_iq.data_ptr()),
-                                  _lib.FLAG_ACCUMULATE if accumulate else 0, self._stream())
+        flags = _lib.FLAG_ACCUMULATE if accumulate else 0
+        if deterministic:
+            nb = int(self.lib.rca_adjoint_det_workspace_bytes(ctypes.byref(self.desc), nf))
+            ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+            rc = self.lib.rca_adjoint_det(ctypes.byref(self.desc), ctypes.c_void_p(grad_out.data_ptr()),
+                                          gt, nf, ctypes.c_void_p(grad_iq.data_ptr()), flags,
+                                          ctypes.c_void_p(ws.data_ptr()), nb, self._stream())
+        else:
+            rc = self.lib.rca_adjoint(ctypes.byref(self.desc), ctypes.c_void_p(grad_out.data_ptr()), gt,
+                                      nf, ctypes.c_void_p(grad_iq.data_ptr()), flags, self._stream())
         _lib.check(rc)
         return grad_iq
 

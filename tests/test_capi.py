@@ -51,6 +51,26 @@ def test_validation_errors_without_gpu():
     assert rc == _lib.RCA_ERR_INVALID
 
 
+def test_deterministic_adjoint_validation_without_gpu():
+    """rca_adjoint_det: workspace size (256 B header + 16 B per complex gradient
+    sample) and the argument checks that return before any GPU work."""
+    lib = _lib.load()
+    d = _lib.RcaDasDesc()
+    d.n_pix, d.n_channel, d.n_tx, d.n_sample = 10, 4, 3, 64
+    assert lib.rca_adjoint_det_workspace_bytes(ctypes.byref(d), 5) == 256 + 16 * 5 * 3 * 4 * 64
+    assert lib.rca_adjoint_det_workspace_bytes(ctypes.byref(d), -1) == 0
+    assert lib.rca_adjoint_det_workspace_bytes(None, 1) == 0
+    need = lib.rca_adjoint_det_workspace_bytes(ctypes.byref(d), 2)
+    rc = lib.rca_adjoint_det(ctypes.byref(d), None, 1, 2, None, 0, None, need, None)
+    assert rc == _lib.RCA_ERR_INVALID and b"workspace" in lib.rca_last_error()
+    rc = lib.rca_adjoint_det(ctypes.byref(d), None, 1, 2, None, 0, ctypes.c_void_p(0x1008), need, None)
+    assert rc == _lib.RCA_ERR_INVALID and b"aligned" in lib.rca_last_error()
+    rc = lib.rca_adjoint_det(ctypes.byref(d), None, 1, 2, None, 0, ctypes.c_void_p(0x1000), need - 1, None)
+    assert rc == _lib.RCA_ERR_INVALID and b"needed" in lib.rca_last_error()
+    rc = lib.rca_adjoint_det(ctypes.byref(d), None, 1, 2, None, 0, ctypes.c_void_p(0x1000), need, None)
+    assert rc == _lib.RCA_ERR_INVALID   # null grid pointers (check_desc), before any launch
+
+
 def test_svd_validation_without_gpu():
     lib = _lib.load()
     assert lib.rca_svd_workspace_bytes(0, 4) == 0
