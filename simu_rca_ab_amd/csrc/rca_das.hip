@@ -1069,7 +1069,7 @@ __device__ __forceinline__ void flush_add(const KArgs& a, double dsc, long long 
 template <int KIND, int FB, bool PT, int V>
 __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, float* rw,
                                             int* claim, unsigned act, int nt, float er, float ei,
-                                            int ri, float rf, int cb, bool last, int step, int k,
+                                            int ri, float rf, int cb, bool last, int step,
                                             int wcap, int nf, const float2* g, int f0, int t0,
                                             long long p, bool ab_tc, const int* s_uab_i,
                                             const float* s_uab_f,
@@ -1077,17 +1077,26 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
   constexpr int LO = Interp<KIND>::LO, HI = Interp<KIND>::HI, NT = LO + HI + 1;
   const int ns = a.n_sample;
   const int lane = threadIdx.x & 63;
+  // updates of this pass: first tap base = m - LO in [cb, hi) (the range
+  // test of .cu:323 included), as t = base - cb < span (unsigned)
+  const int hi = last ? ns - NT : min(ns - NT, cb + step);
+  const unsigned span = hi > cb ? (unsigned)(hi - cb) : 0u;
+  const unsigned rio = (unsigned)(ri - LO - cb);
 #pragma unroll 1
   for (int j = 0; j < nt; ++j, tv += kThreads * V, rw += FB * 2 * wcap) {
     if (!((act >> j) & 1u)) continue;
+    // .cu:310-319: u = (bi + bf) + (ri + rf) [+ the per-tilt law], m = floor(u)
     const TxTerm x = *tv;
-    int m;
-    float d;
-    if (ab_tc) sample_pos(x.bi + ri + s_uab_i[j], x.bf + rf + s_uab_f[j], m, d);
-    else sample_pos(x.bi + ri, x.bf + rf, m, d);                 // .cu:310-319
-    if (!(m >= LO && m < ns - 1 - HI)) continue;                  // .cu:323
-    const int base = m - LO;
-    if (!rca_tile::in_pass(base, cb, last, step)) continue;
+    float f = x.bf + rf;
+    unsigned tb = (unsigned)x.bi + rio;
+    if (ab_tc) {
+      f += s_uab_f[j];
+      tb += (unsigned)s_uab_i[j];
+    }
+    const float fl = floorf(f);
+    const unsigned t = tb + (unsigned)(int)fl;   // m - LO - cb
+    if (t >= span) continue;
+    const float d = f - fl;
     float q[NT];
     interp_weights<KIND>(d, q);
     float pr = er * x.et.x - ei * x.et.y;
@@ -1100,7 +1109,7 @@ __device__ __forceinline__ void adj_scatter(const KArgs& a, const TxTerm* tv, fl
     }
     // the bounds put every tap inside the row; the clamp only keeps LDS
     // accesses inside it
-    const int off = min(max(base - cb, 0), wcap - NT);
+    const int off = (int)min(t, (unsigned)(wcap - NT));
     float2* rv = reinterpret_cast<float2*>(rw) + off;
     float br[FB], bi[FB];
 #pragma unroll
@@ -1305,11 +1314,11 @@ __device__ __forceinline__ void adj_block(const KArgs& a, unsigned bx, unsigned 
           const TxTerm* tv = txc + v * kThreads + tid;
           if (gt1)
             adj_scatter<KIND, FB, false, V>(a, tv, myrows, claim, act[v], nt, er, ei, ri, rf, cb, last,
-                                            step, k, wcap, nf, g[v], f0, t0, p[v], ab_tc, s_uab_i,
+                                            step, wcap, nf, g[v], f0, t0, p[v], ab_tc, s_uab_i,
                                             s_uab_f, s_eab);
           else
             adj_scatter<KIND, FB, true, V>(a, tv, myrows, claim, act[v], nt, er, ei, ri, rf, cb, last,
-                                           step, k, wcap, nf, g[v], f0, t0, p[v], ab_tc, s_uab_i,
+                                           step, wcap, nf, g[v], f0, t0, p[v], ab_tc, s_uab_i,
                                            s_uab_f, s_eab);
         }
         lds_barrier();
@@ -1504,11 +1513,11 @@ __device__ __forceinline__ void adj_block_lo(const KArgs& a, unsigned bx, unsign
             const TxTerm* tv = txc + j * 64 + lane;
             if (gt1)
               adj_scatter<KIND, FB, false, V>(a, tv, myrows, claim, 1u, 1, er[v], ei[v], ri[v], rf[v], cb,
-                                              last, step, k, wcap, nf, g[v], f0, t, p[v], ab_tc, s_uab_i,
+                                              last, step, wcap, nf, g[v], f0, t, p[v], ab_tc, s_uab_i,
                                               s_uab_f, s_eab);
             else
               adj_scatter<KIND, FB, true, V>(a, tv, myrows, claim, 1u, 1, er[v], ei[v], ri[v], rf[v], cb,
-                                             last, step, k, wcap, nf, g[v], f0, t, p[v], ab_tc, s_uab_i,
+                                             last, step, wcap, nf, g[v], f0, t, p[v], ab_tc, s_uab_i,
                                              s_uab_f, s_eab);
           }
           lds_barrier();

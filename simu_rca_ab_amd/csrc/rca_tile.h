@@ -152,6 +152,9 @@ __host__ __device__ __forceinline__ int pass_step(int wcap, int NT, int E) {
 __host__ __device__ __forceinline__ int pass_count(int s0, int whi, int wcap, int step) {
   return whi - s0 > wcap ? 1 + (whi - s0 - wcap + step - 1) / step : 1;
 }
+// (The kernels test the range of .cu:323 and the pass together, as one
+// unsigned compare base - cb < hi - cb with hi = last ? ns - NT :
+// min(ns - NT, cb + step); this is the host emulation's form.)
 __host__ __device__ __forceinline__ bool in_pass(int base, int cb, bool last, int step) {
   return base >= cb && (last || base < cb + step);
 }
