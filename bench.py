@@ -521,14 +521,18 @@ def run_serving(dev, capacity, n_req=16, load=0.7, seed=3000):
     return st
 
 
-def run_serving_saturated(dev, capacity, n_req=12, seed=3100):
+def run_serving_saturated(dev, capacity, n_req=12, seed=3100, frames=(4, 16), host=True,
+                          compare=True):
     """FP16-I/Q serving at saturation: requests drawn like the serving
     workload (grid N_xy ~ U{64..160}, N_z = 2 N_xy, tilts in {8,16,32,64}),
-    4-16 frames each, all FP16, all submitted at once so the queue never runs
-    dry; the sustained rate is compared with the plain FP32 kernel's.  One
+    ``frames`` (inclusive range) frames each, all FP16, all submitted at once
+    so the queue never runs dry; the sustained rate is compared with the
+    plain FP32 kernel's.  I/Q in pinned host memory (uploaded by the server)
+    or, with host=False, resident in HBM at arrival.  With ``compare`` one
     request is then served again with its complex64 I/Q, and its FP16 result
     is compared with that FP32 result."""
     import numpy as np
+    import torch
     from simu_rca_ab_amd.serving import GpuBackend, RcaServer, RequestSpec, synth_request
     rng = np.random.Generator(np.random.PCG64(seed))
     be = GpuBackend(dev)
@@ -536,10 +540,11 @@ def run_serving_saturated(dev, capacity, n_req=12, seed=3100):
     for _ in range(n_req):
         nxy = int(rng.integers(64, 161))
         specs.append(RequestSpec(nxy, nxy, 2 * nxy, int(rng.choice([8, 16, 32, 64])),
-                                 int(rng.integers(4, 17)), "fp16"))
-    log(f"[bench] saturated FP16 serving: synthesising {n_req} requests")
-    reqs = [synth_request(i, sp, be, dev, n_scat=100) for i, sp in enumerate(specs)]
-    srv = RcaServer(be, budget_bytes=48 << 30).start()
+                                 int(rng.integers(frames[0], frames[1] + 1)), "fp16"))
+    log(f"[bench] saturated FP16 serving ({frames[0]}-{frames[1]} frames): synthesising {n_req} requests")
+    reqs = [synth_request(i, sp, be, dev, n_scat=100, host=host) for i, sp in enumerate(specs)]
+    torch.cuda.synchronize()
+    srv = RcaServer(be, budget_bytes=(48 if host else 160) << 30).start()
     log(f"[bench] saturated FP16 serving: {n_req} requests submitted at once")
     for r in reqs:
         srv.submit(r)
@@ -547,6 +552,14 @@ def run_serving_saturated(dev, capacity, n_req=12, seed=3100):
         r.wait(600)
     srv.close(60)
     st = srv.stats.summary()
+    st.update({"plain_fp32_updates_per_s": capacity, "vs_plain_fp32": st["updates_per_s"] / capacity,
+               "n_requests": n_req, "frames_per_request": f"{frames[0]}-{frames[1]}", "dtype": "fp16",
+               "arrival": "all at t=0 (saturation)",
+               "iq": "pinned host memory, uploaded by the server" if host else "resident in HBM at arrival",
+               "latency": "arrival -> beamformed volume in host memory"})
+    if not compare:
+        del reqs
+        return st
     s0 = specs[0]
     twin = synth_request(0, RequestSpec(s0.nx, s0.ny, s0.nz, s0.n_tilt, s0.n_frame, "fp32"), be, dev,
                          n_scat=100)
@@ -556,12 +569,8 @@ def run_serving_saturated(dev, capacity, n_req=12, seed=3100):
     srv2.close(60)
     a, b = reqs[0].result, twin.result
     rel = float((a - b).abs().norm() / b.abs().norm())
-    st.update({"plain_fp32_updates_per_s": capacity, "vs_plain_fp32": st["updates_per_s"] / capacity,
-               "fp16_vs_fp32_rel_l2": rel, "rel_l2_request": f"request 0 ({s0.nx}x{s0.ny}x{s0.nz}, "
-                                                          f"{s0.n_tilt} tilts, {s0.n_frame} frames)",
-               "n_requests": n_req, "frames_per_request": "4-16", "dtype": "fp16",
-               "arrival": "all at t=0 (saturation)",
-               "latency": "arrival -> beamformed volume in host memory"})
+    st.update({"fp16_vs_fp32_rel_l2": rel, "rel_l2_request": f"request 0 ({s0.nx}x{s0.ny}x{s0.nz}, "
+                                                          f"{s0.n_tilt} tilts, {s0.n_frame} frames)"})
     return st
 
 
@@ -744,6 +753,10 @@ def main():
         torch.cuda.empty_cache()
         log("[bench] saturated FP16 serving leg")
         extra["serving_fp16_saturated"] = run_serving_saturated(dev, value)
+        torch.cuda.empty_cache()
+        log("[bench] saturated FP16 serving leg, frames as configs[3]")
+        extra["serving_fp16_saturated_1_100_frames"] = run_serving_saturated(
+            dev, value, n_req=6, seed=3200, frames=(1, 100), host=False, compare=False)
         torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
