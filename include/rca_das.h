@@ -89,7 +89,7 @@ typedef struct {
   int n_pix;
   int n_channel;
   int n_tx;
-  int n_sample;              /* I/Q samples per trace */
+  int n_sample;              /* I/Q samples per trace, at most 2^27 */
   int receive_x;             /* flagReceiveX */
   int mask_transmit;         /* flagMaskTransmit */
   int interp;                /* rca_interp */

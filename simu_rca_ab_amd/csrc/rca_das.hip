@@ -57,6 +57,7 @@ using rca_tile::FwdLayout;
 using rca_tile::TileBox;
 using rca_tile::TxTerm;
 using rca_tile::kMaxLds;
+using rca_tile::kMaxSamples;
 using rca_tile::kThreads;
 using rca_tile::kTileCols;
 using rca_tile::pass_count;
@@ -357,7 +358,10 @@ __device__ __forceinline__ TileBox tile_voxels(const KArgs& a, unsigned bx, int 
 
 // Transmit sample offset stored for a (voxel, tilt) outside the transmit
 // cone when OFF: far beyond any trace, so every sample position built on it
-// fails the forward's range test (real offsets stay within +-2^30 samples).
+// fails the forward's range test.  Real offsets are clamped to +-2^28
+// (split_pos), so the sentinel plus a receive offset and a per-tilt law
+// offset is at least 2^29 - 2^27 - NT, above every trace length (<= 2^27),
+// and at most 2^30 + 2^29 + 3, with no 32-bit wrap.
 constexpr int kTxOff = 1 << 30;
 
 // Transmit terms of tilts [t0, t0 + nt) for the lane's voxels into the LDS
@@ -1608,6 +1612,9 @@ int check_desc(const rca_das_desc* d) {
   if (d->n_pix < 0 || d->n_channel < 1 || d->n_tx < 1 || d->n_sample < 1)
     return fail(RCA_ERR_INVALID, "bad sizes: nPix=%d nChannel=%d nTx=%d nSample=%d", d->n_pix,
                 d->n_channel, d->n_tx, d->n_sample);
+  if (d->n_sample > kMaxSamples)
+    return fail(RCA_ERR_UNSUPPORTED, "nSample=%d exceeds %d samples per trace", d->n_sample,
+                kMaxSamples);
   if (d->n_pix > 0 && (!d->grid_x || !d->grid_y || !d->grid_z))
     return fail(RCA_ERR_INVALID, "grid pointers are NULL");
   if (!d->ele_x || !d->ele_y || !d->ele_z) return fail(RCA_ERR_INVALID, "element pointers are NULL");

@@ -23,6 +23,9 @@ namespace rca_tile {
 
 constexpr int kThreads = 256;
 constexpr int kMaxLds = 64 * 1024;
+// Longest trace (samples) the kernels accept: sample offsets are clamped to
+// +-2^28 and summed in 32-bit unsigned arithmetic (split_pos).
+constexpr int kMaxSamples = 1 << 27;
 
 // Forward tiles: columns per wave (the 4 waves stacked in depth), measured
 // best of 4, 8, 16 on 128x128x256 volumes.
@@ -162,12 +165,17 @@ __host__ __device__ __forceinline__ bool in_pass(int base, int cb, bool last, in
 // x = i + f with i = floor(x) and f = x - i in [0, 1], computed in FP64.
 // Sample positions are summed from such parts in integer + FP32 arithmetic:
 // the fractions are exact to 2^-25 samples each, far below what the weights
-// and the range test resolve.  (Offsets beyond +-2^30 samples are clamped;
-// such updates are out of range anyway.)
+// and the range test resolve.  Offsets beyond +-2^28 samples (and NaN) become
+// exactly +-2^28 (NaN: -2^28) with f = 0: out of every trace (traces hold at
+// most 2^27 samples), and a sum of up to three such integer parts plus the
+// transmit sentinel stays inside 32 bits, which the kernels' unsigned range
+// test relies on.
+constexpr double kPosClamp = 268435456.0;   // 2^28
 __host__ __device__ __forceinline__ void split_pos(double x, int& i, float& f) {
-  const double fl = fmin(fmax(floor(x), -1073741824.0), 1073741824.0);
-  i = (int)fl;
-  f = (float)(x - fl);
+  const double fl = floor(x);
+  const double cl = fmin(fmax(fl, -kPosClamp), kPosClamp);
+  i = (int)cl;
+  f = fl == cl ? (float)(x - fl) : 0.0f;
 }
 
 // m = floor(u), d = u - m for u = (bi + bf) + (ri + rf) [+ (ai + af)].

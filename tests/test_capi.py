@@ -44,6 +44,10 @@ def test_validation_errors_without_gpu():
     d.n_pix, d.n_channel, d.n_tx, d.n_sample = 10, 4, 2, 64
     rc = lib.rca_forward(ctypes.byref(d), None, 0, None, 1, None, None, 0, None)
     assert rc == _lib.RCA_ERR_INVALID   # null grid pointers
+    d.n_sample = (1 << 27) + 1          # sample offsets are 32-bit with a 2^28 clamp
+    rc = lib.rca_forward(ctypes.byref(d), None, 0, None, 1, None, None, 0, None)
+    assert rc == _lib.RCA_ERR_UNSUPPORTED and b"nSample" in lib.rca_last_error()
+    d.n_sample = 64
     with pytest.raises(_lib.RcaError):
         _lib.set_tuning(3, 0, 0)
     _lib.set_tuning(0, 0, 0)
