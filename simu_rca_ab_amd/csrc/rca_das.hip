@@ -1031,9 +1031,10 @@ __device__ __forceinline__ void lds_barrier() {
 // voxel adds |q w (g.re cos + g.im sin)| <= 2 gmax (weights |q| <= 1, w <= 1),
 // so the bound is 2.25 n_pix gmax (gmax = max over grad_out of max(|re|,
 // |im|)).  The quantum is 2^-S ~ 2.25 n_pix gmax / 2^62, about 1e-12 gmax at
-// config 4.  S = 62 - e with 2^(e-1) <= bound < 2^e; scale 1 when gmax is 0.
+// config 4.  S = 62 - e with 2^(e-1) <= bound < 2^e; scale 1 when gmax is 0
+// or not finite (det_to_float then writes NaN whatever the sums hold).
 __host__ __device__ __forceinline__ double det_scale(float gmax, long long n_pix, bool inverse) {
-  if (!(gmax > 0.0f) || n_pix < 1) return 1.0;
+  if (!(gmax > 0.0f) || !(gmax <= 3.4028235e38f) || n_pix < 1) return 1.0;
   const int e = ilogb(2.25 * (double)n_pix * (double)gmax) + 1;
   return ldexp(1.0, inverse ? e - 62 : 62 - e);
 }
@@ -1804,6 +1805,16 @@ int launch_adj_k(const KArgs& a, int FB, int V, hipStream_t st) {
 // (ws non-null: fixed-point sums in ws, then converted into grad_iq).
 constexpr size_t kDetHeader = 256;
 
+// Interpolation taps NT = LO + HI + 1 of a (validated) kind.
+int taps_of(int interp) {
+  switch (interp) {
+    case RCA_INTERP_QUADRATIC: return Interp<RCA_INTERP_QUADRATIC>::LO + Interp<RCA_INTERP_QUADRATIC>::HI + 1;
+    case RCA_INTERP_LANCZOS_REF: return Interp<RCA_INTERP_LANCZOS_REF>::LO + Interp<RCA_INTERP_LANCZOS_REF>::HI + 1;
+    case RCA_INTERP_LANCZOS3: return Interp<RCA_INTERP_LANCZOS3>::LO + Interp<RCA_INTERP_LANCZOS3>::HI + 1;
+    default: return Interp<RCA_INTERP_LANCZOS4>::LO + Interp<RCA_INTERP_LANCZOS4>::HI + 1;
+  }
+}
+
 int adjoint_run(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_frame,
                 void* grad_iq, int flags, void* ws, hipStream_t stream) {
   g_err.clear();
@@ -1848,7 +1859,9 @@ int adjoint_run(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_
                        reinterpret_cast<float*>(grad_iq), n);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "fill_zero launch");
   }
-  if (d->n_pix > 0) {
+  // a trace of at most NT samples holds no update (.cu:323 needs m - LO >= 0
+  // and m + HI < nSample - 1): A^H grad_out is zero, which the fill above wrote
+  if (d->n_pix > 0 && d->n_sample > taps_of(d->interp)) {
     switch (d->interp) {
       case RCA_INTERP_QUADRATIC: rc = launch_adj_k<RCA_INTERP_QUADRATIC>(a, fb, v, stream); break;
       case RCA_INTERP_LANCZOS_REF: rc = launch_adj_k<RCA_INTERP_LANCZOS_REF>(a, fb, v, stream); break;

@@ -54,7 +54,7 @@ def test_det_adjoint_matches_oracle_and_float_adjoint(dev, per_tilt, tiled):
 
 def test_det_adjoint_accumulate_zero_and_nonfinite(dev):
     cfg, plan, G, T = _small(dev, nx=5, ny=4)
-    say("case deterministic adjoint: accumulate, zero grad_out, NaN grad_out")
+    say("case deterministic adjoint: accumulate, zero grad_out, NaN and Inf grad_out")
     g = torch.Generator(device=dev).manual_seed(72)
     y = torch.randn((2, 1, plan.n_pix), dtype=torch.complex64, device=dev, generator=g)
     base = torch.randn((2, plan.n_tx, plan.n_channel, plan.n_sample), dtype=torch.complex64,
@@ -71,6 +71,11 @@ def test_det_adjoint_accumulate_zero_and_nonfinite(dev):
     yn = y.clone()
     yn[1, 0, 3] = complex(float("nan"), 0.0)
     n = plan.adjoint(yn, deterministic=True)
+    torch.cuda.synchronize()
+    assert torch.isnan(torch.view_as_real(n)).all()
+    yi = y.clone()
+    yi[0, 0, 5] = complex(0.0, float("inf"))
+    n = plan.adjoint(yi, deterministic=True)
     torch.cuda.synchronize()
     assert torch.isnan(torch.view_as_real(n)).all()
 

@@ -149,6 +149,22 @@ def test_lds_budgets(kib):
     _ok(W.check(cfg, 1, lds_kib=kib, **LAYOUTS["adjoint_v1"]), f"cfg1 40x40x256 adjoint V=1 LDS {kib} KiB")
 
 
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_short_traces(layout):
+    """Traces of as many samples as taps hold no in-range update; 24-sample
+    traces whose start time is moved to the shallowest depth hold a few, all
+    inside the rows (the shapes of the GPU short-trace tests)."""
+    for interp, nt in (("quadratic", 3), ("lanczos3", 6), ("lanczos4", 8)):
+        for half in (0, 1):
+            cfg = C.scaled(C.get("cfg1"), nx=5, ny=4, nz=64, n_sample=nt)
+            r = W.check(cfg, half, interp=interp, **LAYOUTS[layout])
+            print(f"{interp} half {half} {nt} samples", {k: r[k] for k in KEYS}, flush=True)
+            assert r["updates_in_range"] == 0 and r["violations"] == 0
+            cfg = C.scaled(C.get("cfg1"), nx=5, ny=4, nz=64, n_sample=24, t0=-6.4e-6)
+            _ok(W.check(cfg, half, interp=interp, **LAYOUTS[layout]),
+                f"{interp} half {half} 24 samples from 6.4 us {layout}")
+
+
 def test_no_depth_hint_whole_trace_rows():
     """Without the hint a workgroup owns 256 consecutive voxels and rows hold
     whole traces from the window start."""
