@@ -36,6 +36,9 @@ sys.path.insert(0, str(ROOT))
 os.environ.setdefault("MIOPEN_FIND_MODE", "FAST")
 
 HBM_PEAK_GBS = 8000.0
+# MI355X L2: 8 XCDs x 16 channels x 128 B per clock at 2.4 GHz (derived from
+# the L2 organisation; no published figure)
+L2_PEAK_GBS = 8 * 16 * 128 * 2.4
 # MI355X FP32 vector peak 157.3 TFLOP/s = 78.6 T lane-FMA/s (2 FLOP per FMA)
 VALU_PEAK_NOMINAL_T = 78.6
 TRAFFIC_PROFILE = ROOT / "profiles" / "das_fwd_cfg1_counters.md"
@@ -103,6 +106,15 @@ def roofline(cfg, mean_launch_ms, upd_launch, probe_t):
     # per launch, mean of the two halves: I/Q and grid read, output and mask
     # written (the second half also reads them to accumulate)
     compulsory = iq + 12 * n_pix + 1.5 * (8 * n_pix + 4 * n_pix)
+    # L2 (TCC) traffic: read requests x 128 B + write requests x 64 B, the
+    # request sizes a 512 MiB copy calibrates
+    l2_rd, l2_wr = _profile_value("l2_read_bytes_per_launch"), _profile_value("l2_write_bytes_per_launch")
+    l2 = None
+    if l2_rd and l2_wr:
+        l2 = {"achieved_gbs": (l2_rd + l2_wr) / t / 1e9, "peak_gbs": L2_PEAK_GBS,
+              "frac": (l2_rd + l2_wr) / t / 1e9 / L2_PEAK_GBS, "read_bytes": l2_rd, "write_bytes": l2_wr,
+              "hit_rate": _profile_value("l2_hit_rate"),
+              "peak_source": "8 XCDs x 16 channels x 128 B/clock x 2.4 GHz (derived)"}
     return {"bound": "valu", "achieved": achieved, "peak": probe_t, "unit": "T VALU lane-instr/s",
             "frac": achieved / probe_t if achieved else None, "traffic": traffic,
             "peak_source": "measured in this run: rca_probe_valu mode 1 (8 independent v_fma_f32 "
@@ -113,13 +125,14 @@ def roofline(cfg, mean_launch_ms, upd_launch, probe_t):
                            "about half that rate (ceiling_two_vgpr_sources, probe mode 0)",
             "peak_nominal": VALU_PEAK_NOMINAL_T, "frac_of_nominal": achieved / VALU_PEAK_NOMINAL_T if achieved else None,
             "valu_insts_per_launch": vinst, "valu_lane_instr_per_update": vinst * 64 / upd_launch if vinst else None,
-            "source": "profiles/das_fwd_cfg1_counters.md (SQ_INSTS_VALU, FETCH_SIZE, WRITE_SIZE)",
+            "source": "profiles/das_fwd_cfg1_counters.md (SQ_INSTS_VALU, FETCH_SIZE, WRITE_SIZE, TCC_READ, TCC_WRITE, TCC_HIT, TCC_MISS)",
             "kernel": "das_fwd<lanczos4, FP32, V=1, FB=1>", "launch_ms": mean_launch_ms,
             "updates_per_launch": upd_launch,
             "hbm": {"achieved_gbs": traffic / t / 1e9 if traffic else None, "peak_gbs": HBM_PEAK_GBS,
                     "frac": traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None,
                     "compulsory_bytes": compulsory,
                     "traffic_over_compulsory": traffic / compulsory if traffic else None},
+            "l2": l2,
             "lds_reuse_factor": algo / traffic if traffic else None,
             "interp_footprint_bytes_per_update": cfg.taps() * 8}
 
