@@ -11,7 +11,8 @@ own sizes:
 * config 1 (128+128, 2 x 16 tilts, 128x128x256, a=4): whole workgroup tiles at
   the lateral corners, edges and centre, first / middle / last depth tile,
   against the oracle; the whole volume against the untiled launch, per tilt
-  and compounded, both halves;
+  and compounded, both halves; the adjoint of both halves (dot-product
+  identity, tiled vs untiled, gradient traces against the oracle adjoint);
 * config 2 (config-1 geometry, 200-frame Doppler ensemble, 40 dB tissue
   slab): FP16-I/Q against the FP32 kernel on all 200 frames, both against the
   oracle on whole tiles of frames 0, 99 and 199;
@@ -133,6 +134,51 @@ def test_cfg1_tiled_vs_untiled_full_volume(dev, cfg1_iq, hi):
         say(f"compound={compound}: rel-L2 {r:.3e}, mask {rm:.3e}, envelope {e:.2e} dB")
         assert r <= (4e-6 if compound else 1e-6) and rm <= 4e-6 and e <= 1e-3
         del o_t, o_f, m_t, m_f
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("hi", [0, 1])
+def test_cfg1_adjoint_full(dev, cfg1_iq, hi):
+    """The adjoint at config 1's full size (the receive-X half runs the
+    line-outer kernel, the receive-Y half the general one): the dot-product
+    identity over the whole volume, the tiled launch against the untiled one,
+    and gradient traces against the oracle adjoint restricted to their tilt
+    and line."""
+    cfg = C.get("cfg1")
+    say(f"case cfg1 half {hi}: full-size adjoint (128x128x256, 16 tilts, 128 lines)")
+    x = cfg1_iq[hi].to(dev)
+    plan, G, T = tiled_plan(cfg, hi, dev)
+    y_fwd, _ = plan.forward(x, want_mask=False)
+    g = torch.Generator(device=dev).manual_seed(1100 + hi)
+    yv = torch.randn((1, 1, plan.n_pix), dtype=torch.complex64, device=dev, generator=g)
+    AHy = plan.adjoint(yv)
+    torch.cuda.synchronize()
+    lhs = torch.vdot(yv.flatten().to(torch.complex128), y_fwd.flatten().to(torch.complex128))
+    rhs = torch.vdot(AHy.flatten().to(torch.complex128), x.flatten().to(torch.complex128))
+    r = abs(complex(lhs - rhs)) / abs(complex(lhs))
+    say(f"dot-product identity at full size: mismatch {r:.3e}")
+    assert r <= REL
+    flat, _, _ = tiled_plan(cfg, hi, dev, depth_run=0)
+    r = H.rel_l2_t(AHy, flat.adjoint(yv))
+    torch.cuda.synchronize()
+    say(f"tiled vs untiled adjoint: rel-L2 {r:.3e}")
+    assert r <= REL
+    prep = O.prepare(G, T)
+    grid = H.grid_f32(G)
+    prm = H.oracle_params(cfg, cfg.halves[hi])
+    yc = yv.cpu().numpy()
+    for t, c in ((0, 0), (8, 64), (15, 127)):
+        p1 = dict(prep)
+        for k in ("ex", "ey", "ez"):
+            p1[k] = np.ascontiguousarray(prep[k][c:c + 1])
+        for k in ("vx", "vy", "vz", "d0"):
+            p1[k] = np.ascontiguousarray(prep[k][t:t + 1])
+        want = O.kernel_adjoint(yc, grid, p1, prm, 1, O.INTERP_LANCZOS, 4)[:, 0, 0]
+        have = AHy[:, t, c].cpu().numpy()
+        r = H.rel_l2(have, want)
+        say(f"gradient trace (tilt {t}, line {c}): rel-L2 vs oracle adjoint {r:.3e}")
+        assert r <= REL
+    del plan, flat, AHy, y_fwd
     torch.cuda.empty_cache()
 
 
