@@ -429,10 +429,20 @@ class RcaServer:
                 if self.in_use + need > self.budget:
                     return
                 self.queue.popleft()
+                req.admitted = time.perf_counter()
+                try:
+                    ev = self.be.upload(req)
+                except Exception as exc:   # e.g. out of device memory: this request fails
+                    req.error = f"upload of request {req.rid} failed: {exc!r}"
+                    req.done = time.perf_counter()
+                    try:
+                        self.be.release(req)
+                    except Exception:
+                        pass
+                    req.event.set()
+                    continue
                 self.in_use += need
                 req.dev_bytes = need
-                req.admitted = time.perf_counter()
-                ev = self.be.upload(req)
                 req.dev["uploaded"] = ev
                 self.active.append(req)
                 self.stats.max_in_flight = max(self.stats.max_in_flight, len(self.active))
@@ -549,6 +559,14 @@ class RcaServer:
             with self.lock:
                 self.error = repr(exc)
                 self._closed = True
+            # batches still on the GPU use the requests' device buffers: let
+            # them end before the buffers are released
+            for j in (0, 1):
+                if self.slots[j] is not None:
+                    try:
+                        self.slots[j][0].synchronize()
+                    except Exception:
+                        pass
             self._stop_downloads()
             self._fail_pending(f"server failed: {exc!r}")
 

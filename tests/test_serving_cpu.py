@@ -25,17 +25,20 @@ class FakeBackend:
     the result of frame f of request r is r*1000 + f in every voxel."""
     n_elem = 4
 
-    def __init__(self, upd_per_s=1e9, fail_on_batch=None):
+    def __init__(self, upd_per_s=1e9, fail_on_batch=None, fail_upload_of=None):
         self.rate = upd_per_s
         self.busy_until = 0.0
         self.batches = []
         self.fail_on_batch = fail_on_batch
+        self.fail_upload_of = fail_upload_of
         self.live = {}
 
     def request_bytes(self, spec):
         return spec.n_vox * spec.n_frame * 8
 
     def upload(self, req):
+        if req.rid == self.fail_upload_of:
+            raise RuntimeError("injected out of memory")
         req.dev = {"out": np.zeros((req.spec.n_frame, req.spec.n_vox), np.complex64)}
         self.live[req.rid] = True
         return _Ev(time.perf_counter() + 1e-3)
@@ -140,6 +143,22 @@ def test_backend_failure_reaches_waiters():
     assert "injected launch failure" in srv.error
     with pytest.raises(RuntimeError, match="server failed"):
         srv.submit(_req(1))
+    srv.close(5)
+
+
+def test_upload_failure_fails_that_request_only():
+    """An upload that raises (out of device memory) fails its own request and
+    returns its budget; the server goes on with the others."""
+    be = FakeBackend(fail_upload_of=1)
+    srv = RcaServer(be, budget_bytes=10**9, quantum=10**6).start()
+    reqs = [srv.submit(_req(i, frames=2)) for i in range(3)]
+    with pytest.raises(RuntimeError, match="injected out of memory"):
+        reqs[1].wait(5)
+    for i in (0, 2):
+        out = reqs[i].wait(5)
+        assert np.all(out[1] == i * 1000 + 1)
+    srv.drain(5)
+    assert srv.in_use == 0 and srv.error is None
     srv.close(5)
 
 
