@@ -678,9 +678,9 @@ def main():
     ap.add_argument("--no-sim", action="store_true", help="skip the SIMUS-RCA simulator leg")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the NoSum / aberration-law variants leg")
-    ap.add_argument("--dbf", action="store_true",
-                    help="add the differentiable-beamforming leg (its first run compiles MIOpen "
-                         "Conv3d kernels, which can take minutes)")
+    ap.add_argument("--no-dbf", action="store_true",
+                    help="skip the differentiable-beamforming leg (config 4 DAS + complex CNN "
+                         "train steps; its first run compiles MIOpen Conv3d kernels)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device copy (HBM-counter calibration)")
     ap.add_argument("--valu-probe", action="store_true",
@@ -756,7 +756,7 @@ def main():
         log("[bench] SIMUS-RCA simulator leg (one half-frame)")
         extra["simulator"] = run_sim(dev)
         torch.cuda.empty_cache()
-    if args.dbf:
+    if not args.no_dbf:
         log("[bench] differentiable beamforming leg")
         extra["dbf"] = run_dbf(dev)
         torch.cuda.empty_cache()
