@@ -208,12 +208,56 @@ __device__ __forceinline__ float horner_monic(float u) {
   return horner_tail<KIND, P, ODD, D - 2>(u, add_lit<coef_bits<KIND, P, ODD>(D - 1)>(u));
 }
 
+// lead * M(u) (LEAD) or M(u) as one asm statement.  The compiler's hazard
+// check treats every inline asm as a possible 16-bit dst_sel forwarder and
+// puts a wait state between an asm result and its next-instruction consumer;
+// plain 32-bit VALU results need none, so a chain written as one statement
+// issues without those s_nop (14 -> 1 per update in the config-1 forward,
+// +0.7 % measured).  Same instructions in the same order as horner_monic
+// (+ mul_lit): bitwise the same weights.
+template <int KIND, int P, bool ODD, bool LEAD>
+__device__ __forceinline__ float horner_block(float u) {
+  constexpr int D = InterpPoly<KIND>::kDeg;
+  if constexpr (D != 3 && D != 4) {
+    const float m = horner_monic<KIND, P, ODD>(u);
+    return LEAD ? mul_lit<lead_bits<KIND, P, ODD>()>(m) : m;
+  } else {
+    float r;
+    if constexpr (D == 3 && LEAD) {
+      asm("v_add_f32 %0, %2, %1\n\tv_fmaak_f32 %0, %1, %0, %3\n\tv_fmaak_f32 %0, %1, %0, %4\n\t"
+          "v_mul_f32 %0, %5, %0"
+          : "=&v"(r)
+          : "v"(u), "i"(coef_bits<KIND, P, ODD>(2)), "i"(coef_bits<KIND, P, ODD>(1)),
+            "i"(coef_bits<KIND, P, ODD>(0)), "i"(lead_bits<KIND, P, ODD>()));
+    } else if constexpr (D == 3) {
+      asm("v_add_f32 %0, %2, %1\n\tv_fmaak_f32 %0, %1, %0, %3\n\tv_fmaak_f32 %0, %1, %0, %4"
+          : "=&v"(r)
+          : "v"(u), "i"(coef_bits<KIND, P, ODD>(2)), "i"(coef_bits<KIND, P, ODD>(1)),
+            "i"(coef_bits<KIND, P, ODD>(0)));
+    } else if constexpr (LEAD) {
+      asm("v_add_f32 %0, %2, %1\n\tv_fmaak_f32 %0, %1, %0, %3\n\tv_fmaak_f32 %0, %1, %0, %4\n\t"
+          "v_fmaak_f32 %0, %1, %0, %5\n\tv_mul_f32 %0, %6, %0"
+          : "=&v"(r)
+          : "v"(u), "i"(coef_bits<KIND, P, ODD>(3)), "i"(coef_bits<KIND, P, ODD>(2)),
+            "i"(coef_bits<KIND, P, ODD>(1)), "i"(coef_bits<KIND, P, ODD>(0)),
+            "i"(lead_bits<KIND, P, ODD>()));
+    } else {
+      asm("v_add_f32 %0, %2, %1\n\tv_fmaak_f32 %0, %1, %0, %3\n\tv_fmaak_f32 %0, %1, %0, %4\n\t"
+          "v_fmaak_f32 %0, %1, %0, %5"
+          : "=&v"(r)
+          : "v"(u), "i"(coef_bits<KIND, P, ODD>(3)), "i"(coef_bits<KIND, P, ODD>(2)),
+            "i"(coef_bits<KIND, P, ODD>(1)), "i"(coef_bits<KIND, P, ODD>(0)));
+    }
+    return r;
+  }
+}
+
 // q_p and q_{1-p} from the even and odd parts of pair p.
 template <int KIND, int P>
 __device__ __forceinline__ void weight_pair(float e, float u, float* q) {
   constexpr int LO = Interp<KIND>::LO;
-  const float pe = mul_lit<lead_bits<KIND, P, false>()>(horner_monic<KIND, P, false>(u));
-  const float mo = horner_monic<KIND, P, true>(u);
+  const float pe = horner_block<KIND, P, false, true>(u);
+  const float mo = horner_block<KIND, P, true, false>(u);
   const float eo = mul_lit<lead_bits<KIND, P, true>()>(e);
   q[P + LO] = fmaf(eo, mo, pe);
   q[1 - P + LO] = fmaf(-eo, mo, pe);
