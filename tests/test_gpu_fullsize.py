@@ -13,6 +13,8 @@ own sizes:
   against the oracle; the whole volume against the untiled launch, per tilt
   and compounded, both halves; the adjoint of both halves (dot-product
   identity, tiled vs untiled, gradient traces against the oracle adjoint);
+* config 1 with each aberration law (per line, per (tilt, line), per (line,
+  voxel) with 2 x 2.1 GB of delays): whole tiles against the oracle;
 * config 2 (config-1 geometry, 200-frame Doppler ensemble, 40 dB tissue
   slab): FP16-I/Q against the FP32 kernel on all 200 frames, both against the
   oracle on whole tiles of frames 0, 99 and 199;
@@ -180,6 +182,58 @@ def test_cfg1_adjoint_full(dev, cfg1_iq, hi):
         assert r <= REL
     del plan, flat, AHy, y_fwd
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("kind,hi", [("channel", 0), ("tx_channel", 1), ("pixel_channel", 0)])
+def test_cfg1_laws_whole_tiles(dev, cfg1_iq, kind, hi):
+    """The aberration-law kernels at config 1's full size through the
+    depth-tiled launch: per line (Hyp1), per (tilt, line) (Hyp2) and per
+    (line, voxel) with receive and transmit arrays (TxRx, 2 x 2.1 GB), delays
+    of up to 3 samples, random per entry; whole workgroup tiles (corners,
+    edges, centre x first/middle/last depth tile) vs the FP64 oracle."""
+    cfg = C.get("cfg1")
+    n_ch, n_tx, n_pix = cfg.n_elem, len(cfg.halves[hi].theta), cfg.n_vox
+    say(f"case cfg1 half {hi}: {kind} law, full size, depth-tiled, whole tiles vs oracle")
+    rng = np.random.default_rng(310 + hi)
+    span = 3.0 / cfg.fs
+
+    def draw(shape):
+        a = rng.random(shape, dtype=np.float32)
+        a *= 2 * span
+        a -= span
+        return a
+
+    sel, _ = H.edge_tiles(cfg.nx, cfg.ny, cfg.nz, n_random=4, seed=31)
+    if kind == "channel":
+        rx, tx = draw(n_ch), None
+        okw = dict(ablaw=rx, ablaw_mode=1)
+    elif kind == "tx_channel":
+        rx, tx = draw((n_ch, n_tx)), None
+        okw = dict(ablaw=rx, ablaw_mode=2)
+    else:
+        rx, tx = draw((n_pix, n_ch)), draw((n_pix, n_ch))   # MATLAB [nPix, nChannel]
+        okw = dict(ablaw=np.ascontiguousarray(rx[sel]), ablaw_tx=np.ascontiguousarray(tx[sel]),
+                   ablaw_mode=3)
+    G, T, P, Op = H.case(cfg, hi)
+    P = dict(P, AbLawRx=rx, AbLawTx=tx, AbLawMode=kind)
+    plan = RcaPlan.from_reference(H.flat_geometry(G), T, P, Op, n_sample=cfg.n_sample, device=dev,
+                                  depth_run=cfg.nz)
+    del rx, tx
+    assert plan.depth_run == cfg.nz
+    out, mask = plan.forward(cfg1_iq[hi].to(dev))
+    torch.cuda.synchronize()
+    got = out[0, 0].cpu().numpy()[sel]
+    gm = mask[0].cpu().numpy()[sel]
+    del plan, out, mask
+    torch.cuda.empty_cache()
+    Gs = dict(G)
+    for k in ("gridBfX", "gridBfY", "gridBfZ"):
+        Gs[k] = np.asarray(G[k]).ravel(order="F")[sel]
+    kind_o, a = O.INTERP_LANCZOS, 4
+    r, m = O.kernel_forward(np.ascontiguousarray(cfg1_iq[hi].numpy()).view(np.float32), H.grid_f32(Gs),
+                            O.prepare(G, T), H.oracle_params(cfg, cfg.halves[hi]), kind_o, a, **okw)
+    check(got, r.sum(axis=1)[0], f"cfg1 half {hi} {kind} law, {sel.size} voxels in whole tiles")
+    assert H.rel_l2(gm, m.sum(axis=0)) <= REL
 
 
 def test_cfg2_ensemble_200_frames(dev):
