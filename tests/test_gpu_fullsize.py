@@ -14,7 +14,9 @@ own sizes:
   and compounded, both halves; the adjoint of both halves (dot-product
   identity, tiled vs untiled, gradient traces against the oracle adjoint);
 * config 1 with each aberration law (per line, per (tilt, line), per (line,
-  voxel) with 2 x 2.1 GB of delays): whole tiles against the oracle;
+  voxel) with 2 x 2.1 GB of delays): whole tiles against the oracle; NoSum
+  on a 64-deep 128 x 32 slab (4.3 GB of per-line outputs): whole tiles
+  against the oracle, the line sum against the per-tilt output;
 * config 2 (config-1 geometry, 200-frame Doppler ensemble, 40 dB tissue
   slab): FP16-I/Q against the FP32 kernel on all 200 frames, both against the
   oracle on whole tiles of frames 0, 99 and 199;
@@ -234,6 +236,43 @@ def test_cfg1_laws_whole_tiles(dev, cfg1_iq, kind, hi):
                             O.prepare(G, T), H.oracle_params(cfg, cfg.halves[hi]), kind_o, a, **okw)
     check(got, r.sum(axis=1)[0], f"cfg1 half {hi} {kind} law, {sel.size} voxels in whole tiles")
     assert H.rel_l2(gm, m.sum(axis=0)) <= REL
+
+
+@pytest.mark.parametrize("hi", [0, 1])
+def test_cfg1_nosum_slab(dev, cfg1_iq, hi):
+    """NoSum (one output per receive line, NoSum.cu:376-380) at the bench's
+    variant shape: a 64-deep 128 x 32 slab of the config-1 grid, 16 tilts x
+    128 lines (4.3 GB of per-line output), depth-tiled: whole tiles vs the
+    oracle, and the per-line outputs summed over the lines equal to the
+    per-tilt output over the whole slab."""
+    cfg = C.get("cfg1")
+    h = cfg.halves[hi]
+    nx, ny, nz = cfg.nx, 32, 64
+    say(f"case cfg1 half {hi}: NoSum on a {nz}-deep {nx}x{ny} slab, 16 tilts x 128 lines")
+    x, y, z = C.grid_axes(cfg)
+    G = C.geometry(cfg, h, C.meshgrid_zxy(x, y[:ny], z[:nz]))
+    T = C.transmit(h)
+    plan = RcaPlan.from_reference(H.flat_geometry(G), T, C.param(cfg), C.options(cfg),
+                                  n_sample=cfg.n_sample, device=dev, depth_run=nz)
+    assert plan.depth_run == nz
+    xi = cfg1_iq[hi].to(dev)
+    ns, _ = plan.forward(xi, compound=False, nosum=True, want_mask=False)   # [nCh, F, T, P]
+    pt, _ = plan.forward(xi, compound=False, want_mask=False)               # [F, T, P]
+    torch.cuda.synchronize()
+    r = H.rel_l2_t(ns.sum(dim=0), pt)
+    say(f"sum over lines of the NoSum outputs vs the per-tilt output: rel-L2 {r:.3e}")
+    assert r <= 1e-6
+    sel, _ = H.edge_tiles(nx, ny, nz, n_random=4, seed=17)
+    got = ns[:, 0][..., torch.from_numpy(sel).to(dev)].cpu().numpy()        # [nCh, T, n_sel]
+    del ns, pt, plan
+    torch.cuda.empty_cache()
+    Gs = dict(G)
+    for k in ("gridBfX", "gridBfY", "gridBfZ"):
+        Gs[k] = np.asarray(G[k]).ravel(order="F")[sel]
+    ref, _ = O.kernel_forward(np.ascontiguousarray(cfg1_iq[hi].numpy()).view(np.float32),
+                              H.grid_f32(Gs), O.prepare(G, T), H.oracle_params(cfg, h),
+                              O.INTERP_LANCZOS, 4, nosum=True)
+    check(got, ref[:, 0], f"cfg1 half {hi} NoSum slab, {sel.size} voxels x 16 tilts x 128 lines")
 
 
 def test_cfg2_ensemble_200_frames(dev):
