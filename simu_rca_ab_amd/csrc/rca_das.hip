@@ -1875,15 +1875,42 @@ int adjoint_run(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_
   a.gin = reinterpret_cast<float2*>(grad_iq);
   a.gout_tx = grad_tx;
   a.accumulate = (flags & RCA_FLAG_ACCUMULATE) != 0;
-  // frames per workgroup and voxels per lane: FB = 4 with V = 4 measured
-  // fastest on a 96x96x192 volume (8 frames); fewer frames take V = 2, which
-  // keeps the kernel at 3 waves per SIMD
+  // Frames per workgroup (FB) and voxels per lane (V).  FB = 4 with V = 4
+  // measured fastest on a 96x96x192 volume (8 frames) and on both config-1
+  // halves (4 and 8 frames).  A partial frame block costs a full one (the
+  // scatter runs all FB frames), so the frames go in launches of
+  // power-of-two blocks: the 4-frame blocks that fit, then a 2-frame and a
+  // 1-frame launch for the rest, on the same fill and workspace.  Those take
+  // V = 2 (3 waves per SIMD), except 2-frame blocks on the receive-x half
+  // with depth tiles, where V = 4 sends the tiles to the line-outer kernel.
+  // Config 1, ms: 2 frames 74.9 (V = 4) vs 80.5 on receive x, 85 vs 113
+  // (V = 4) on receive y; receive y, 3 frames as 2 + 1: 134 vs 161 in one
+  // launch of 2-frame blocks; 5 and 6 frames 167 and 193 vs 234 in 4-frame
+  // blocks (receive x).  A tuned FB or V (rca_set_tuning) gives one launch
+  // of that shape, as does the pixel law (its one instantiated shape).
+  struct Part { int f0, nf, fb, v; };
+  Part parts[3];
+  int n_parts = 0;
   const int tfb = g_tune_fb.load(), tv = g_tune_v.load();
-  int fb = tfb ? tfb : 4;
-  while (fb > 1 && fb > n_frame) fb >>= 1;
-  int v = tv ? tv : (fb == 4 ? 4 : 2);
-  if (fb == 8 && v > 2) v = 2;
-  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) fb = v = 1;   // the instantiated pixel-law shape
+  const bool lo_tiles = d->receive_x != 0 && d->depth_run >= 64;
+  if (d->ab_law_mode == RCA_ABLAW_PIXEL_CHANNEL) {
+    parts[n_parts++] = {0, n_frame, 1, 1};
+  } else if (tfb || tv) {
+    int fb = tfb ? tfb : 4;
+    while (fb > 1 && fb > n_frame) fb >>= 1;
+    int v = tv ? tv : ((fb == 4 || (fb == 2 && lo_tiles)) ? 4 : 2);
+    if (fb == 8 && v > 2) v = 2;
+    parts[n_parts++] = {0, n_frame, fb, v};
+  } else if (n_frame % 4 == 3 && lo_tiles) {
+    // three frames left on the line-outer half: one partial 4-frame block
+    // (118 ms) beats 2 + 1 (124 ms); on the other half 2 + 1 wins (134 vs 147)
+    parts[n_parts++] = {0, n_frame, 4, 4};
+  } else {
+    const int n4 = n_frame / 4 * 4;
+    if (n4) parts[n_parts++] = {0, n4, 4, 4};
+    if (n_frame - n4 >= 2) parts[n_parts++] = {n4, 2, 2, lo_tiles ? 4 : 2};
+    if ((n_frame - n4) % 2) parts[n_parts++] = {n_frame - 1, 1, 1, 2};
+  }
   const long long n = 2LL * n_frame * d->n_tx * d->n_channel * d->n_sample;   // floats of grad_iq
   hipError_t e;
   if (ws) {
@@ -1906,14 +1933,25 @@ int adjoint_run(const rca_das_desc* d, const void* grad_out, int grad_tx, int n_
   // a trace of at most NT samples holds no update (.cu:323 needs m - LO >= 0
   // and m + HI < nSample - 1): A^H grad_out is zero, which the fill above wrote
   if (d->n_pix > 0 && d->n_sample > taps_of(d->interp)) {
-    switch (d->interp) {
-      case RCA_INTERP_QUADRATIC: rc = launch_adj_k<RCA_INTERP_QUADRATIC>(a, fb, v, stream); break;
-      case RCA_INTERP_LANCZOS_REF: rc = launch_adj_k<RCA_INTERP_LANCZOS_REF>(a, fb, v, stream); break;
-      case RCA_INTERP_LANCZOS3: rc = launch_adj_k<RCA_INTERP_LANCZOS3>(a, fb, v, stream); break;
-      case RCA_INTERP_LANCZOS4: rc = launch_adj_k<RCA_INTERP_LANCZOS4>(a, fb, v, stream); break;
-      default: return fail(RCA_ERR_INVALID, "unknown interpolation %d", d->interp);
+    const long long trace_samples = (long long)d->n_tx * d->n_channel * d->n_sample;
+    for (int k = 0; k < n_parts; ++k) {
+      const Part& q = parts[k];
+      // the part's frames: grad_out, grad_iq and the fixed-point sums from
+      // frame q.f0 on (the kernels index frames from their base pointers)
+      KArgs b = a;
+      b.n_frame = q.nf;
+      b.gout = a.gout + (long long)q.f0 * grad_tx * d->n_pix;
+      b.gin = a.gin + (long long)q.f0 * trace_samples;
+      if (a.gdet) b.gdet = a.gdet + 2 * (long long)q.f0 * trace_samples;
+      switch (d->interp) {
+        case RCA_INTERP_QUADRATIC: rc = launch_adj_k<RCA_INTERP_QUADRATIC>(b, q.fb, q.v, stream); break;
+        case RCA_INTERP_LANCZOS_REF: rc = launch_adj_k<RCA_INTERP_LANCZOS_REF>(b, q.fb, q.v, stream); break;
+        case RCA_INTERP_LANCZOS3: rc = launch_adj_k<RCA_INTERP_LANCZOS3>(b, q.fb, q.v, stream); break;
+        case RCA_INTERP_LANCZOS4: rc = launch_adj_k<RCA_INTERP_LANCZOS4>(b, q.fb, q.v, stream); break;
+        default: return fail(RCA_ERR_INVALID, "unknown interpolation %d", d->interp);
+      }
+      if (rc) return rc;
     }
-    if (rc) return rc;
   }
   if (ws) {
     hipLaunchKernelGGL(det_to_float, dim3(4096), dim3(256), 0, stream,
