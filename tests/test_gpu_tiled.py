@@ -141,6 +141,33 @@ def test_tiled_fp16_and_nosum(dev):
     assert r <= 1e-6
 
 
+@pytest.mark.parametrize("nf", [6, 7])
+@pytest.mark.parametrize("half_index", [0, 1])
+def test_adjoint_frame_split(dev, nf, half_index):
+    """Without a tuned shape the adjoint runs the frames in launches of
+    4-, 2- and 1-frame blocks (6 = 4 + 2; 7 = 4 + 2 + 1, or one launch of
+    4-frame blocks on the receive-x half with depth tiles): every frame of
+    the split equals that frame's own one-frame adjoint, for the float-atomic
+    and the fixed-point sums, and with accumulation."""
+    say(f"case adjoint frame split nf={nf} half={half_index}")
+    _, tiled, G = plans(ROWS, half_index, dev, "lanczos4")
+    rng = np.random.default_rng(61 + nf + half_index)
+    g = torch.from_numpy((rng.standard_normal((nf, 1, tiled.n_pix)) +
+                          1j * rng.standard_normal((nf, 1, tiled.n_pix))).astype(np.complex64)).to(dev)
+    one = torch.cat([tiled.adjoint(g[f:f + 1].contiguous()) for f in range(nf)])
+    for det in (False, True):
+        r = H.rel_l2_t(tiled.adjoint(g, deterministic=det), one)
+        say(f"deterministic={det}: {nf} frames vs one frame at a time: rel-L2 {r:.3e}")
+        assert r <= 1e-6
+    base = torch.from_numpy((rng.standard_normal(tuple(one.shape)) +
+                             1j * rng.standard_normal(tuple(one.shape))).astype(np.complex64)).to(dev)
+    acc = base.clone()
+    tiled.adjoint(g, grad_iq=acc, accumulate=True)
+    r = H.rel_l2_t(acc, base + one)
+    say(f"accumulated: rel-L2 vs base + one-frame adjoints {r:.3e}")
+    assert r <= 1e-6
+
+
 @pytest.mark.parametrize("interp", ["quadratic", "lanczos4"])
 @pytest.mark.parametrize("half_index", [0, 1])
 def test_tiled_adjoint(dev, interp, half_index):
