@@ -159,14 +159,15 @@ template <> struct Interp<RCA_INTERP_LANCZOS4> { static constexpr int LO = 3, HI
 // add (u + a_top, one register operand), then v_fmaak_f32 (p * u + a_j); the
 // leads fold into one multiply for the even part and one for e of the odd
 // part, and each tap is one FMA: q_{p,1-p} = fma(+-e*leadO, M_O, leadE*M_E).
-// A pair costs 2 adds, 2 (DEG - 1) FMAs, 2 multiplies and 2 FMAs: 36
-// instructions for the 8 taps of a = 4 (half of them with one register
+// A pair costs 2 adds, 2 (DEG - 1) FMAs, 2 multiplies and 2 FMAs: 40
+// instructions for the 8 taps of a = 4 (16 of them with one register
 // operand, which issue at twice the rate of two- and three-operand ones on
 // gfx950), against 46 for the degree-4 Horner form with the leading
 // coefficient moved into a register and 95 for the sine-identity form with
 // one reciprocal per update.  Left to itself the compiler keeps the
 // loop-invariant constants in SGPRs, and an FMA with an SGPR source issues at
-// half rate: hence the literals, in inline asm.
+// half rate: hence the literals, in inline asm (the multiply of e by the odd
+// lead stays plain C, which gets its literal from the compiler).
 template <unsigned K>
 __device__ __forceinline__ float fma_lit(float x, float y) {
   float r;
@@ -258,7 +259,9 @@ __device__ __forceinline__ void weight_pair(float e, float u, float* q) {
   constexpr int LO = Interp<KIND>::LO;
   const float pe = horner_block<KIND, P, false, true>(u);
   const float mo = horner_block<KIND, P, true, false>(u);
-  const float eo = mul_lit<lead_bits<KIND, P, true>()>(e);
+  // plain C: the compiler gives it the literal operand by itself, and an asm
+  // result here would cost the next instruction a hazard wait state
+  const float eo = e * InterpPoly<KIND>::kOddLead[P - 1];
   q[P + LO] = fmaf(eo, mo, pe);
   q[1 - P + LO] = fmaf(-eo, mo, pe);
   if constexpr (P > 1) weight_pair<KIND, P - 1>(e, u, q);
