@@ -764,12 +764,16 @@ def main():
         log("[bench] serving leg (config 3)")
         extra["serving"] = run_serving(dev, ens_rate or value)
         torch.cuda.empty_cache()
-        log("[bench] saturated FP16 serving leg")
-        extra["serving_fp16_saturated"] = run_serving_saturated(dev, value)
-        torch.cuda.empty_cache()
-        log("[bench] saturated FP16 serving leg, frames as configs[3]")
-        extra["serving_fp16_saturated_1_100_frames"] = run_serving_saturated(
+        # the FP16 serving mode at saturation on the serving workload's own
+        # frame counts (1-100 per request), I/Q resident at arrival
+        log("[bench] saturated FP16 serving leg (frames 1-100)")
+        extra["serving_fp16_saturated"] = run_serving_saturated(
             dev, value, n_req=6, seed=3200, frames=(1, 100), host=False, compare=False)
+        torch.cuda.empty_cache()
+        # small requests (4-16 frames) uploaded from pinned host memory, and
+        # FP16 vs FP32 on one of them
+        log("[bench] saturated FP16 serving leg (small requests, host I/Q)")
+        extra["serving_fp16_saturated_small_requests"] = run_serving_saturated(dev, value)
         torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
