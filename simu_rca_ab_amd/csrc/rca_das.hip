@@ -801,11 +801,15 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
               }
               const float fl = floorf(f);
               const unsigned t = tb + (unsigned)(int)fl;
+              // the bounds put every tap inside the row; the clamp only keeps
+              // LDS reads inside it.  Clamped before the range test and held
+              // there by an empty asm: sunk past the branch, t was computed a
+              // second time (one VALU per update; +0.6 % measured)
+              unsigned tcl = min(t, (unsigned)(wcap - NT));
+              __asm__ volatile("" : "+v"(tcl));
               if (t >= span) continue;
               const float d = f - fl;
-              // the bounds put every tap inside the row; the clamp only keeps
-              // LDS reads inside it
-              const S* trv = rows + min(t, (unsigned)(wcap - NT));
+              const S* trv = rows + tcl;
               // single frame: the taps are read before the weights are
               // evaluated, so their LDS latency overlaps that arithmetic
               // (read after it, the latency was exposed: +10 % measured)
