@@ -780,11 +780,10 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
             sincos2pi((float)(ph - rint(ph)), sn, cs);
             const float er = w * cs, ei = w * sn;
             const unsigned rio = (unsigned)(ri - LO - cb);
-            const TxTerm* tv = txc + v * kThreads + tid;
-            const S* rows = tr;
-#pragma unroll 2
-            for (int j = 0; j < nt; ++j, tv += V * kThreads, rows += FB * wcap) {
-              if (!kOff && !((act[v] >> j) & 1u)) continue;   // .cu:191-214
+            const TxTerm* tv0 = txc + v * kThreads + tid;
+            // one tilt of the chunk: tilt j's transmit terms at tv, rows at rows
+            const auto tilt = [&](int j, const TxTerm* tv, const S* rows) __attribute__((always_inline)) {
+              if (!kOff && !((act[v] >> j) & 1u)) return;   // .cu:191-214
               // .cu:310-319: u = (bi + bf) + (ri + rf) [+ the per-tilt law],
               // m = floor(u), t = m - LO - cb
               const TxTerm x = *tv;
@@ -807,7 +806,7 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
               // second time (one VALU per update; +0.6 % measured)
               unsigned tcl = min(t, (unsigned)(wcap - NT));
               __asm__ volatile("" : "+v"(tcl));
-              if (t >= span) continue;
+              if (t >= span) return;
               const float d = f - fl;
               const S* trv = rows + tcl;
               // single frame: the taps are read before the weights are
@@ -858,6 +857,21 @@ __device__ __forceinline__ void fwd_block(const KArgs& a, unsigned bx, unsigned 
                     dst.y = fmaf(re, i2, fmaf(im, r2, dst.y));
                   }
                 }
+              }
+            };
+            // the chunk's tilts, fully unrolled to the chunk's most (TC), so
+            // the transmit-term and row offsets are immediates: +1.6 % FP32
+            // and +3.2 % FP16 8-frame blocks, measured (single frame: same).
+            // NoSum kernels store inside the loop and lost a wave per SIMD
+            // that way: they keep the loop.
+            if constexpr (NS) {
+#pragma unroll 2
+              for (int j = 0; j < nt; ++j) tilt(j, tv0 + j * V * kThreads, tr + j * FB * wcap);
+            } else {
+#pragma unroll
+              for (int j = 0; j < TC; ++j) {
+                if (j >= nt) break;
+                tilt(j, tv0 + j * V * kThreads, tr + j * FB * wcap);
               }
             }
           }
